@@ -1,0 +1,249 @@
+/*
+ * saruman_hip.h -- C ABI of libsaruman_hip.so, the MI355X (gfx950) replacement of
+ * SARUMAN's GPU alignment module.
+ *
+ * Part 1 keeps the reference's four entry points and the globals they read and
+ * write, with the reference's signatures (src/cuda-functions.h:15-32, 55-78), so
+ * the reference's host C (hash.c, saruman.c, util.c) links against this library
+ * unchanged.  Part 2 is the extended batch API the batched, graph and serving
+ * paths use: per-candidate read length and error bound, results in arrays
+ * instead of printf, explicit error codes, an explicit stream per context.
+ *
+ * No torch types, no HIP types: plain pointers and sizes only.
+ */
+#ifndef SARUMAN_HIP_H_
+#define SARUMAN_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== */
+/* Part 1. Drop-in entry points (reference: src/cuda-functions.h:55-78)   */
+/* ===================================================================== */
+
+/*
+ * Replaces cuda_needleman_wunsch, src/cuda-functions.h:55-63 (definition
+ * src/cuda-functions.cu:584-1049).  Same arguments, same synchronous behaviour:
+ * aligns sample_size candidates laid out as prepare_cuda_alignments builds them
+ * (src/hash.c:898-1111), prints one TSV record per hit (score <= error) to stdout
+ * (src/cuda-functions.cu:987-994), sets forward_/reverse_output_control[read_id],
+ * and increments *_alignments_done / *_alignment_candidates.  Every input array
+ * is borrowed.  edit_offsets and total_edit_matrix_size are accepted and ignored
+ * (the library sizes its own scratch).  On an internal error it prints a message
+ * to stderr and returns without output, as the reference does when a host
+ * allocation fails (src/cuda-functions.cu:626-638).
+ */
+void cuda_needleman_wunsch(int match, int mismatch, char *genome_sequences_on_host,
+                           char *read_sequences_on_host, int *genome_offsets, int *read_offsets,
+                           int *edit_offsets, int sample_size, int *read_id_array,
+                           int *genome_pos_array, char *direction_array, int error,
+                           size_t total_genome_array_size, size_t total_read_sequence_array_size,
+                           size_t total_edit_matrix_size, int do_reverse);
+
+/*
+ * Replaces cuda_allocate_memory, src/cuda-functions.h:65-67 (definition
+ * src/cuda-functions.cu:541-562).  Creates the per-strand context (HIP stream and
+ * device pool) for do_reverse == 4 (reverse) or any other value (forward) on GPU
+ * use_gpu_value.  The pool grows on demand and is capped by cuda_memory_per_thread
+ * when that global is non-zero; larger batches are split (the reference never
+ * checked, Q19).
+ */
+void cuda_allocate_memory(int do_reverse);
+
+/*
+ * Replaces cuda_free_memory, src/cuda-functions.h:69-71 (definition
+ * src/cuda-functions.cu:564-575): do_reverse != 0 frees the reverse context,
+ * 0 the forward one.
+ */
+void cuda_free_memory(int do_reverse);
+
+/*
+ * Replaces get_cuda_infos, src/cuda-functions.h:77-78 (definition
+ * src/cuda-functions.cu:1051-1120).  Lists the GPUs on stderr, picks the one with
+ * the most free memory unless use_gpu_value names a valid device, and returns the
+ * free bytes of the device that will be used (the reference returned the best
+ * device's even when another was forced, Q22).  Exits with status 1 below
+ * 128,000,000 free bytes, as the reference does.
+ */
+unsigned long int get_cuda_infos(void);
+
+/* ---- globals of the drop-in contract ---------------------------------- */
+/* Defined by the reference's host program (src/util.h, src/saruman.h,
+ * src/hash.h tentative definitions).  libsaruman_hip.so also defines them so it
+ * loads stand-alone; in a process whose executable defines them, the dynamic
+ * linker binds the library's references to the executable's copies. */
+
+/* uthash 1.9.4 handle layout (src/uthash.h), needed only to walk read_names */
+typedef struct srm_ut_hash_handle {
+    void *tbl;
+    void *prev;
+    void *next;      /* next element in insertion order */
+    void *hh_prev;
+    void *hh_next;
+    void *key;
+    unsigned keylen;
+    unsigned hashv;
+} srm_ut_hash_handle;
+
+/* src/util.h:62-66 */
+struct read_name_hash {
+    int read_number;
+    char name[60];
+    srm_ut_hash_handle hh;
+};
+
+extern struct read_name_hash *read_names;          /* util.h:55, read by name */
+extern unsigned short int read_length_value;       /* util.h:25 */
+extern unsigned short int be_verbose_value;        /* util.h:26 */
+extern unsigned short int use_gpu_value;           /* util.h:24, written by get_cuda_infos */
+extern unsigned long int cuda_memory_per_thread;   /* saruman.h:60 */
+extern char *forward_output_control;               /* saruman.h:55 */
+extern char *reverse_output_control;               /* saruman.h:56 */
+extern unsigned int forward_alignments_done;       /* saruman.h:62 */
+extern unsigned int reverse_alignments_done;       /* saruman.h:63 */
+extern unsigned int forward_alignment_candidates;  /* saruman.h:65 */
+extern unsigned int reverse_alignment_candidates;  /* saruman.h:66 */
+extern char *main_alloc_ptr_for;                   /* cuda-functions.h:31 */
+extern char *main_alloc_ptr_rev;                   /* cuda-functions.h:32 */
+extern float alignment_time[2];                    /* cuda-functions.cu:61 (ms, fwd/rev) */
+
+/* ===================================================================== */
+/* Part 2. Extended batch API (no counterpart in the reference)           */
+/* ===================================================================== */
+
+/* error codes */
+#define SRM_OK 0
+#define SRM_EINVAL -1      /* malformed argument or candidate */
+#define SRM_ENOMEM -2      /* host or device allocation failed / over budget */
+#define SRM_EDEVICE -3     /* HIP runtime error */
+#define SRM_ENODEV -4      /* no usable GPU */
+#define SRM_ESTATE -5      /* call out of order */
+
+/* kernel selection for srm_job_run */
+#define SRM_KERNEL_AUTO 0    /* banded where eligible, plain otherwise */
+#define SRM_KERNEL_PLAIN 1   /* one alignment per thread, full int16 matrix */
+#define SRM_KERNEL_BANDED 2  /* persistent banded kernel (ineligible candidates -> plain) */
+
+/* srm_cand.bits */
+#define SRM_CAND_STOP_ROW0 1u   /* reproduce the first-candidate traceback stop (Q1) */
+#define SRM_CAND_BYTES 2u       /* offsets are byte offsets into the byte arrays */
+
+/* limits (documented in DESIGN.md) */
+#define SRM_MAX_L 1000
+#define SRM_MAX_W 900
+#define SRM_BAND_MAX 64      /* widest band the banded kernel holds in registers */
+#define SRM_BAND_KMAX 14     /* saturated band values must fit a nibble: k + 1 <= 15 */
+
+/*
+ * One candidate alignment.  The read is given in the orientation it is aligned
+ * in (reverse-complemented for flags 4..7, as prepare_cuda_alignments does,
+ * src/hash.c:1013-1034).  The window holds rows 0..w-1 of the DP matrix.
+ * 2-bit codes: A=0 C=1 G=2 T=3, 16 codes per uint32, code j of a word in bits
+ * 2j..2j+1.  Code arrays must carry at least SRM_CODE_PAD codes of readable
+ * padding before offset 0 and after the last code referenced.
+ */
+typedef struct srm_cand {
+    uint32_t read_off;   /* code index (or byte offset with SRM_CAND_BYTES) of read[0] */
+    uint32_t win_off;    /* code index (or byte offset) of window row 0 */
+    uint32_t ops_off;    /* uint32 index of the transcript slot, >= srm_ops_words(w, k) words */
+    uint16_t L;          /* read length, 1..SRM_MAX_L */
+    uint16_t w;          /* window length, 0..SRM_MAX_W */
+    uint8_t flag;        /* direction_array value; selects s, e (cuda-functions.cu:383-410) */
+    uint8_t k;           /* error bound */
+    uint8_t bits;        /* SRM_CAND_* */
+    uint8_t pad;
+} srm_cand;
+
+#define SRM_CODE_PAD 256
+
+/*
+ * Per-candidate result.  score: M[w][L] for the plain kernel; the banded kernel
+ * saturates scores above k at k+1.  Transcript: n_ops 2-bit operations in
+ * traceback order (first op = the move out of cell (w, L)), SRM_OP_DIAG /
+ * SRM_OP_TOP / SRM_OP_LEFT; the walk stopped at (end_row, end_col).  n_ops == 0
+ * when score > k.  status != 0 flags a candidate the kernel rejected.
+ */
+typedef struct srm_result {
+    int16_t score;
+    uint16_t n_ops;
+    uint16_t end_row;
+    uint16_t end_col;
+} srm_result;
+
+#define SRM_OP_DIAG 0u   /* emits (read[col-1], window[row-1]) */
+#define SRM_OP_TOP 1u    /* emits ('_', window[row-1]) */
+#define SRM_OP_LEFT 2u   /* emits (read[col-1], '_') */
+
+typedef struct srm_ctx srm_ctx;
+typedef struct srm_job srm_job;
+
+/* library/version */
+const char *srm_version(void);
+int srm_device_count(void);
+int srm_device_sync(int device);
+const char *srm_last_error(void);
+
+/* transcript slot size in uint32 words for a window of w rows and bound k */
+static inline uint32_t srm_ops_words(uint32_t w, uint32_t k) { return (w + k + 15u) / 16u; }
+
+/* context = one HIP stream + device pool on one GPU */
+int srm_ctx_create(int device, size_t pool_cap_bytes, srm_ctx **out);
+void srm_ctx_destroy(srm_ctx *ctx);
+
+/*
+ * Keep a packed genome resident on the device (uploaded once).  words/n_words
+ * include the caller's padding; jobs created with win_words == NULL take their
+ * windows from it (srm_cand.win_off = code index into it).
+ */
+int srm_ctx_set_genome(srm_ctx *ctx, const uint32_t *words, size_t n_words);
+
+/*
+ * A job: one batch of candidates, uploaded once, runnable many times.
+ * win_words == NULL -> windows come from the context's resident genome.
+ * read_bytes / win_bytes serve the SRM_CAND_BYTES candidates (may be NULL).
+ */
+int srm_job_create(srm_ctx *ctx, srm_job **out);
+void srm_job_destroy(srm_job *job);
+int srm_job_upload(srm_job *job, int match, int mismatch, const srm_cand *cand, int n,
+                   const uint32_t *read_words, size_t n_read_words, const uint32_t *win_words,
+                   size_t n_win_words, const char *read_bytes, size_t n_read_bytes,
+                   const char *win_bytes, size_t n_win_bytes);
+/* launch the alignment of every candidate on the context's stream; asynchronous */
+int srm_job_run(srm_job *job, int kernel);
+/* wait for the job; copy results (n) and transcript words (ops, may be NULL) */
+int srm_job_download(srm_job *job, srm_result *res, uint32_t *ops, size_t n_ops_words);
+/* device time (ms) of the last srm_job_run, measured with HIP events on the job's stream */
+float srm_job_last_ms(srm_job *job);
+/* per-kernel device time (ms) of the last run: kind 0 = banded, 1 = plain */
+float srm_job_kernel_ms(srm_job *job, int kind);
+/* capture srm_job_run(job, kernel) into a hipGraph and replay it; 0 = plain launches */
+int srm_job_use_graph(srm_job *job, int enable);
+/* count of candidates per path after the last upload: 0 banded, 1 plain(codes), 2 plain(bytes) */
+int srm_job_path_count(srm_job *job, int path);
+
+/*
+ * Record text for one hit: the post-processing of src/cuda-functions.cu:828-994
+ * applied to a candidate whose slot rows are reconstructed from its transcript.
+ * read/win are the candidate's bytes (read oriented as aligned).  Writes one
+ * '\n'-terminated line, returns its length (snprintf style) or < 0 on error.
+ */
+int srm_format_record(const char *name, unsigned int pos, int flag, int k, int L, int w,
+                      const srm_result *res, const uint32_t *ops, const char *read,
+                      const char *win, char *out, int outcap);
+
+/* slot rows (as the reference's kernel leaves them) from a transcript; cap = w + k */
+int srm_slot_rows(const srm_result *res, const uint32_t *ops, const char *read, const char *win,
+                  char *R, char *G, int cap);
+
+/* pack ASCII ACGT into 2-bit codes; returns the number of non-ACGT bytes seen
+ * (those are packed as A) */
+long long srm_pack_codes(const char *seq, size_t n, uint32_t *words, size_t code_off);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SARUMAN_HIP_H_ */

@@ -1,0 +1,53 @@
+// nw_device.h -- argument blocks shared by the kernels (nw_kernels.hip) and the
+// host runtime (nw_runtime.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "saruman_hip.h"
+
+// value seen by the traceback for matrix cells outside the candidate's own matrix
+// (row -1).  The reference reads the previous candidate's matrix there
+// (src/cuda-functions.cu:140,152); this never equals a cell value <= k.
+#define SRM_FOREIGN 30000
+// diag/left at column 0, src/cuda-functions.cu:145-146
+#define SRM_COL0 999
+
+namespace srm {
+
+struct PlainArgs {
+    int ma, mm;
+    int n;                          // candidates in this launch
+    const uint32_t *idx;            // [n] candidate indices
+    const srm_cand *cand;
+    const uint32_t *read_words;     // 2-bit codes (CODES variant)
+    const uint32_t *win_words;
+    const char *read_bytes;         // raw bytes (bytes variant)
+    const char *win_bytes;
+    int16_t *mat;                   // matrix scratch
+    const uint64_t *mat_off;        // [n] element offsets into mat
+    srm_result *res;
+    uint32_t *ops;
+};
+
+struct BandArgs {
+    int ma, mm;
+    const srm_cand *cand;
+    const uint32_t *perm;           // candidate indices, tile-major
+    const uint4 *tiles;             // x perm start, y count (<= 64), z max L, w unused
+    uint32_t ntiles;
+    uint32_t *queue;                // tile counter (zeroed before the launch)
+    const uint32_t *read_words;
+    const uint32_t *win_words;
+    uint32_t *scratch;              // per wave: (maxL + 1) * NW * 64 words
+    size_t scratch_words_per_wave;
+    srm_result *res;
+    uint32_t *ops;
+};
+
+hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st);
+hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
+hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
+
+}  // namespace srm

@@ -1,0 +1,394 @@
+// nw_kernels.hip -- gfx950 kernels for SARUMAN's semiglobal Needleman-Wunsch with
+// traceback (the work of compute_score/get_alignment, src/cuda-functions.cu:76-536).
+//
+// Two kernels compute the same results:
+//   plain_kernel  one candidate per thread, full (w+1)x(L+1) int16 matrix in HBM,
+//                 value-based traceback on that matrix.  Handles every candidate
+//                 (any match/mismatch, raw-byte or 2-bit sequences).
+//   band_kernel   persistent: each wave pulls 64-candidate tiles from a device queue;
+//                 one candidate per lane; the exact k-error band of a DP column
+//                 (d = row - col in [-k, max(2s+k, w-L)]) lives in VGPRs, saturated
+//                 at k+1; each column is stored as nibbles in a per-wave scratch
+//                 ([column][word][lane], coalesced 256-byte stores) and the traceback
+//                 walks those nibbles.  Needs match >= 0, mismatch >= 1, k <= 14.
+//
+// Both reproduce the reference recurrence and traceback decisions bit for bit
+// (quirks Q4-Q7 kept, Q1 selectable per candidate, Q2/Q3 canonical).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nw_device.h"
+
+namespace srm {
+
+// ---------------------------------------------------------------------------
+// sequence access
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t code_at(const uint32_t *__restrict__ words, int64_t idx)
+{
+    return (words[idx >> 4] >> ((uint32_t)(idx & 15) * 2u)) & 3u;
+}
+
+// 16 consecutive codes starting at code index idx (idx may be negative down to
+// -SRM_CODE_PAD: the arrays carry padding)
+__device__ __forceinline__ uint32_t fetch16(const uint32_t *__restrict__ words, int64_t idx)
+{
+    int64_t wi = idx >> 4;
+    uint32_t sh = (uint32_t)(idx & 15) * 2u;
+    uint32_t lo = words[wi];
+    uint32_t hi = words[wi + 1];
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+}
+
+// cuda-functions.cu:383-410: flags 2,3,6,7 free the start (s = k), flags 1,3,5,7
+// free the end (e = k); every other value leaves both at 0
+__device__ __forceinline__ int flag_s(int flag, int k) { return (flag <= 7 && (flag & 2)) ? k : 0; }
+__device__ __forceinline__ int flag_e(int flag, int k) { return (flag <= 7 && (flag & 1)) ? k : 0; }
+
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+// value-based traceback decision, cuda-functions.cu:121-276 (matrix-relative).
+// returns 0 diag, 1 top, 2 left, 3 stop
+__device__ __forceinline__ int decide(int ma, int mm, int v, int Dg, int Lf, int Tp, int wd, int ht,
+                                      int L, int w, int s, int e, bool chars_equal)
+{
+    int corr = ((wd == L - 1 && ht >= w - 2 * e) || (s && wd == 0 && ht <= (s + 1) * 2)) ? mm : 0;
+    if (Dg + mm == v) return 0;
+    if (Dg + ma == v && chars_equal) return 0;
+    if (Tp + mm - corr == v) return 1;
+    if (Lf + mm == v) return 2;
+    return 3;
+}
+
+// ---------------------------------------------------------------------------
+// plain kernel
+// ---------------------------------------------------------------------------
+template <bool CODES>
+struct SeqRef {
+    const uint32_t *words;
+    const char *bytes;
+    int64_t off;
+    __device__ __forceinline__ int at(int i) const
+    {
+        if (CODES) return (int)code_at(words, off + i);
+        return (int)(unsigned char)bytes[off + i];
+    }
+};
+
+template <bool CODES>
+__global__ __launch_bounds__(256) void plain_kernel(PlainArgs a)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.n) return;
+    const uint32_t t = a.idx[j];
+    const srm_cand cd = a.cand[t];
+    const int L = cd.L, w = cd.w, k = cd.k;
+    const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
+    const int ma = a.ma, mm = a.mm;
+    SeqRef<CODES> rd{a.read_words, a.read_bytes, (int64_t)cd.read_off};
+    SeqRef<CODES> wn{a.win_words, a.win_bytes, (int64_t)cd.win_off};
+    int16_t *__restrict__ M = a.mat + a.mat_off[j];
+    const int S = L + 1;
+
+    for (int c = 0; c <= L; ++c) M[c] = (int16_t)c;                     // :413-416
+    {
+        int16_t prev = 0;
+        for (int h = 1; h <= w; ++h) {                                   // :419-430 (Q7)
+            prev = (h < 2 * s + 1) ? prev : (int16_t)(prev + 1);
+            M[h * S] = prev;
+        }
+    }
+    for (int h = 0; h < w; ++h) {                                        // :444-494
+        const int gh = wn.at(h);
+        const int16_t *__restrict__ up = M + h * S;
+        int16_t *__restrict__ cur = M + (h + 1) * S;
+        int16_t diag = up[0];
+        int16_t left = cur[0];
+        const int corr_row = (h >= w - 2 * e) ? mm : 0;
+        for (int c = 0; c < L; ++c) {
+            const int16_t top = up[c + 1];
+            const int corr = (c == L - 1) ? corr_row : 0;
+            const int16_t D = (int16_t)(diag + (gh != rd.at(c) ? mm : ma));
+            const int16_t Lf = (int16_t)(left + mm);
+            const int16_t T = (int16_t)(top + mm - corr);
+            int16_t m = D;                                               // :89-105
+            if (m > Lf) m = Lf;
+            if (m > T) m = T;
+            cur[c + 1] = m;
+            diag = top;
+            left = m;
+        }
+    }
+    const int score = M[w * S + L];
+    srm_result r;
+    r.score = (int16_t)score;
+    r.n_ops = 0;
+    r.end_row = (uint16_t)w;
+    r.end_col = (uint16_t)L;
+    if (score <= k) {                                                    // :504
+        const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
+        uint32_t *__restrict__ ops = a.ops + cd.ops_off;
+        const int cap = w + k;
+        int row = w, col = L, n = 0;
+        uint32_t acc = 0;
+        for (;;) {
+            if (stop_row0 && row == 0) break;                            // Q1
+            const int v = M[row * S + col];
+            int Dg, Lf, Tp;
+            if (col != 0) {
+                Dg = row >= 1 ? M[(row - 1) * S + col - 1] : SRM_FOREIGN;
+                Lf = M[row * S + col - 1];
+            } else {
+                Dg = SRM_COL0;
+                Lf = SRM_COL0;
+            }
+            Tp = row >= 1 ? M[(row - 1) * S + col] : SRM_FOREIGN;
+            const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
+            const bool eq = wn.at(ht) == rd.at(wd);
+            const int op = decide(ma, mm, v, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
+            if (op == 3 || n >= cap) break;
+            acc |= (uint32_t)op << ((n & 15) * 2);
+            ++n;
+            if ((n & 15) == 0) {
+                ops[(n >> 4) - 1] = acc;
+                acc = 0;
+            }
+            row -= (op != 2);
+            col -= (op != 1);
+        }
+        if (n & 15) ops[n >> 4] = acc;
+        r.n_ops = (uint16_t)n;
+        r.end_row = (uint16_t)row;
+        r.end_col = (uint16_t)col;
+    }
+    a.res[t] = r;
+}
+
+// ---------------------------------------------------------------------------
+// banded persistent kernel
+// ---------------------------------------------------------------------------
+// nibble (value) of band cell i in column col, from the wave's scratch
+__device__ __forceinline__ int nib(const uint32_t *__restrict__ scr, int NW, int col, int i, int lane)
+{
+    uint32_t word = scr[((size_t)col * NW + (i >> 3)) * 64 + lane];
+    return (int)((word >> ((i & 7) * 4)) & 15u);
+}
+
+template <int NW, bool UNIT>
+__global__ __launch_bounds__(256) void band_kernel(BandArgs a)
+{
+    constexpr int BMAX = NW * 8;            // band cells per column held in VGPRs
+    constexpr int WW = (BMAX + 15) / 16;    // code words of the sliding window
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    uint32_t *__restrict__ scr = a.scratch + (size_t)wave * a.scratch_words_per_wave;
+    const int ma = a.ma, mm = a.mm;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(a.queue, 1u);
+        tile = __builtin_amdgcn_readfirstlane(tile);
+        if (tile >= a.ntiles) break;
+        const uint4 td = a.tiles[tile];     // x: perm start, y: count, z: max L
+        const bool active = (uint32_t)lane < td.y;
+        const uint32_t t = active ? a.perm[td.x + lane] : 0u;
+        srm_cand cd = a.cand[t];
+        const int L = active ? (int)cd.L : 0;
+        const int w = cd.w, k = cd.k;
+        const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
+        const int dlo = -k;
+        const int K = k + 1;
+        const int Lmax = (int)td.z;
+
+        // ---- column 0: M[h][0] = max(0, h - 2s) (unit steps, Q7); rows < 0 -> K
+        int v[BMAX];
+#pragma unroll
+        for (int i = 0; i < BMAX; ++i) {
+            const int h = dlo + i;
+            v[i] = h < 0 ? K : imin(imax(h - 2 * s, 0), K);
+        }
+        // window codes for column 1: W[i] = g[dlo + i]; a 16*WW-code window plus a stream
+        uint32_t gw[WW];
+        const int64_t gbase = (int64_t)cd.win_off + dlo;
+#pragma unroll
+        for (int j = 0; j < WW; ++j) gw[j] = active ? fetch16(a.win_words, gbase + 16 * j) : 0u;
+        uint32_t gnext = 0;   // stream of window codes entering the top of the window
+        uint32_t rbuf = 0;    // stream of read codes
+        const int64_t rbase = (int64_t)cd.read_off;
+
+        auto store_col = [&](int col) {
+#pragma unroll
+            for (int j = 0; j < NW; ++j) {
+                uint32_t acc = (uint32_t)v[8 * j];
+#pragma unroll
+                for (int q = 1; q < 8; ++q) acc |= (uint32_t)v[8 * j + q] << (4 * q);
+                scr[((size_t)col * NW + j) * 64 + lane] = acc;
+            }
+        };
+        if (active) store_col(0);
+
+        // one DP column; last selects the free trailing verticals of column L
+        auto column = [&](int c, bool last) {
+            // window for column c (c >= 2): shift by one code, take the next from the stream
+            if (c >= 2) {
+                if (((c - 2) & 15) == 0) gnext = fetch16(a.win_words, gbase + 16 * WW + (c - 2));
+#pragma unroll
+                for (int j = 0; j < WW - 1; ++j) gw[j] = __builtin_amdgcn_alignbit(gw[j + 1], gw[j], 2);
+                gw[WW - 1] = __builtin_amdgcn_alignbit(gnext, gw[WW - 1], 2);
+                gnext >>= 2;
+            }
+            if (((c - 1) & 15) == 0) rbuf = fetch16(a.read_words, rbase + (c - 1));
+            const uint32_t rc = rbuf & 3u;
+            rbuf >>= 2;
+            const uint32_t rrep = rc * 0x55555555u;
+            uint32_t mis[WW];
+#pragma unroll
+            for (int j = 0; j < WW; ++j) {
+                const uint32_t x = gw[j] ^ rrep;
+                mis[j] = x | (x >> 1);   // bit 2i set <=> window code i != read code
+            }
+            int ifree = BMAX;            // first band cell whose top move is free
+            if (last) ifree = w - 2 * e + 1 - c - dlo;
+            int up = 1 << 20;            // new value of cell i-1 (out of band above)
+#pragma unroll
+            for (int i = 0; i < BMAX; ++i) {
+                const int bit = (int)((mis[i >> 4] >> (2 * (i & 15))) & 1u);
+                const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 20);
+                int nv;
+                if (UNIT && !last) {
+                    // min(v + bit, lf + 1, up + 1) = min3(v + bit - 1, lf, up) + 1
+                    nv = imin(imin(v[i] + bit - 1, lf), up) + 1;
+                } else {
+                    const int dg = v[i] + (bit ? mm : ma);
+                    const int tcost = (last && i >= ifree) ? 0 : mm;
+                    nv = imin(imin(dg, lf + mm), up + tcost);
+                }
+                nv = imin(nv, K);
+                v[i] = nv;
+                up = nv;
+            }
+            if (!UNIT && c <= k) {
+                // row 0 keeps its unit-step init M[0][c] = c (Q7) whatever the mismatch cost
+                const int i0 = k - c;
+#pragma unroll
+                for (int i = 0; i < BMAX; ++i)
+                    if (i == i0) v[i] = imin(c, K);
+            }
+            store_col(c);
+        };
+
+        for (int c = 1; c < Lmax; ++c) {
+            if (c < L) column(c, false);
+        }
+        if (active) column(L, true);
+
+        // ---- score and traceback (cuda-functions.cu:495-534, App. A.3)
+        if (active) {
+            srm_result r;
+            const int iend = w - L - dlo;
+            int score = K;
+            if (iend >= 0 && iend < BMAX) score = nib(scr, NW, L, iend, lane);
+            r.score = (int16_t)score;
+            r.n_ops = 0;
+            r.end_row = (uint16_t)w;
+            r.end_col = (uint16_t)L;
+            if (score <= k) {
+                const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
+                uint32_t *__restrict__ ops = a.ops + cd.ops_off;
+                const int cap = w + k;
+                int row = w, col = L, n = 0;
+                uint32_t acc = 0;
+                for (;;) {
+                    if (stop_row0 && row == 0) break;
+                    const int i = row - col - dlo;
+                    const int vv = nib(scr, NW, col, i, lane);
+                    int Dg, Lf, Tp;
+                    if (col != 0) {
+                        Dg = row >= 1 ? nib(scr, NW, col - 1, i, lane) : SRM_FOREIGN;
+                        Lf = (i + 1 < BMAX) ? nib(scr, NW, col - 1, i + 1, lane) : K;
+                    } else {
+                        Dg = SRM_COL0;
+                        Lf = SRM_COL0;
+                    }
+                    Tp = row >= 1 ? (i >= 1 ? nib(scr, NW, col, i - 1, lane) : K) : SRM_FOREIGN;
+                    const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
+                    const bool eq = code_at(a.win_words, (int64_t)cd.win_off + ht) ==
+                                    code_at(a.read_words, rbase + wd);
+                    const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
+                    if (op == 3 || n >= cap) break;
+                    acc |= (uint32_t)op << ((n & 15) * 2);
+                    ++n;
+                    if ((n & 15) == 0) {
+                        ops[(n >> 4) - 1] = acc;
+                        acc = 0;
+                    }
+                    row -= (op != 2);
+                    col -= (op != 1);
+                }
+                if (n & 15) ops[n >> 4] = acc;
+                r.n_ops = (uint16_t)n;
+                r.end_row = (uint16_t)row;
+                r.end_col = (uint16_t)col;
+            }
+            a.res[t] = r;
+        }
+    }
+}
+
+// clears the queue counters of a launch sequence (a kernel, not hipMemsetAsync,
+// so the sequence can be captured into a graph and replayed)
+__global__ void reset_counters(uint32_t *c, int n)
+{
+    int i = threadIdx.x;
+    if (i < n) c[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers
+// ---------------------------------------------------------------------------
+hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+    dim3 block(256), grid((a.n + 255) / 256);
+    if (codes)
+        hipLaunchKernelGGL(plain_kernel<true>, grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL(plain_kernel<false>, grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+template <int NW>
+static hipError_t launch_band_nw(const BandArgs &a, bool unit, int blocks, hipStream_t st)
+{
+    dim3 block(256), grid(blocks);
+    if (unit)
+        hipLaunchKernelGGL((band_kernel<NW, true>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((band_kernel<NW, false>), grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st)
+{
+    switch (nw) {
+    case 1: return launch_band_nw<1>(a, unit, blocks, st);
+    case 2: return launch_band_nw<2>(a, unit, blocks, st);
+    case 3: return launch_band_nw<3>(a, unit, blocks, st);
+    case 4: return launch_band_nw<4>(a, unit, blocks, st);
+    case 5: return launch_band_nw<5>(a, unit, blocks, st);
+    case 6: return launch_band_nw<6>(a, unit, blocks, st);
+    case 7: return launch_band_nw<7>(a, unit, blocks, st);
+    case 8: return launch_band_nw<8>(a, unit, blocks, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64 * ((n + 63) / 64)), 0, st, counters, n);
+    return hipGetLastError();
+}
+
+}  // namespace srm

@@ -1,0 +1,864 @@
+// nw_runtime.hip -- host runtime of libsaruman_hip.so: contexts (one HIP stream and
+// device pool each), jobs (one uploaded candidate batch), the launch sequence of
+// the banded and plain kernels, hipGraph replay, and the reference's four C entry
+// points (src/cuda-functions.cu:541-1120) on top of them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "nw_device.h"
+#include "saruman_hip.h"
+
+// ---------------------------------------------------------------------------
+// globals of the drop-in contract (see saruman_hip.h)
+// ---------------------------------------------------------------------------
+extern "C" {
+struct read_name_hash *read_names = nullptr;
+unsigned short int read_length_value = 0;
+unsigned short int be_verbose_value = 0;
+unsigned short int use_gpu_value = 5000;   // util.c:192-194: 5000 = pick automatically
+unsigned long int cuda_memory_per_thread = 0;
+char *forward_output_control = nullptr;
+char *reverse_output_control = nullptr;
+unsigned int forward_alignments_done = 0;
+unsigned int reverse_alignments_done = 0;
+unsigned int forward_alignment_candidates = 0;
+unsigned int reverse_alignment_candidates = 0;
+char *main_alloc_ptr_for = nullptr;
+char *main_alloc_ptr_rev = nullptr;
+float alignment_time[2] = {0.f, 0.f};
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SRM_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct srm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t pool_cap = 0;      // 0 = unlimited
+    size_t pool_used = 0;     // bytes held by this context's buffers
+    int num_cus = 0;
+    DevBuf genome;            // resident packed genome
+    size_t genome_words = 0;
+    std::mutex mu;
+    srm_job *dropin_job = nullptr;
+
+    int ensure(DevBuf &b, size_t bytes)
+    {
+        if (bytes <= b.cap) return SRM_OK;
+        size_t want = std::max(bytes, b.cap + b.cap / 2);
+        want = (want + 4095) & ~(size_t)4095;
+        if (pool_cap && pool_used - b.cap + want > pool_cap) {
+            want = bytes;
+            if (pool_used - b.cap + want > pool_cap)
+                return fail(SRM_ENOMEM, "device pool cap %zu bytes exceeded (need %zu more)",
+                            pool_cap, want);
+        }
+        if (b.p) {
+            hipFree(b.p);
+            pool_used -= b.cap;
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            return fail(SRM_ENOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        }
+        b.cap = want;
+        pool_used += want;
+        return SRM_OK;
+    }
+    void release(DevBuf &b)
+    {
+        if (b.p) {
+            hipFree(b.p);
+            pool_used -= b.cap;
+        }
+        b.p = nullptr;
+        b.cap = 0;
+    }
+};
+
+namespace {
+struct BandClass {
+    int nw = 0;
+    uint32_t tile_begin = 0, ntiles = 0;
+    int maxL = 0;
+    int blocks = 0;
+};
+struct PlainChunk {
+    uint32_t begin = 0, count = 0;   // range in the plain index list
+    size_t mat_elems = 0;
+};
+}  // namespace
+
+struct srm_job {
+    srm_ctx *ctx = nullptr;
+    int n = 0;
+    int ma = 0, mm = 1;
+    bool unit = true;
+    bool resident_genome = false;
+    // device buffers
+    DevBuf d_cand, d_read_words, d_win_words, d_read_bytes, d_win_bytes;
+    DevBuf d_perm, d_tiles, d_queue, d_scratch;
+    DevBuf d_idx_rest, d_idx_all, d_idx_bytes;      // plain index lists
+    DevBuf d_off_rest, d_off_all, d_off_bytes;      // matrix offsets (per list, chunk-relative)
+    DevBuf d_mat, d_res, d_ops;
+    size_t ops_words = 0;
+    // host-side plan
+    std::vector<BandClass> classes;
+    std::vector<PlainChunk> chunks_rest, chunks_all, chunks_bytes;
+    size_t n_band = 0, n_rest = 0, n_bytes = 0;
+    size_t mat_budget_elems = 0;
+    // timing
+    hipEvent_t ev[6] = {};
+    float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f;
+    // graph
+    bool graph_enabled = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_kernel = -1;
+    bool uploaded = false;
+};
+
+namespace {
+
+// band height for the banded kernel: d = row - col in [-k, max(2s+k, w-L)]
+inline int band_cells(const srm_cand &c)
+{
+    int k = c.k;
+    int s = (c.flag <= 7 && (c.flag & 2)) ? k : 0;
+    int hi = std::max(2 * s + k, (int)c.w - (int)c.L);
+    return hi + k + 1;
+}
+
+void destroy_graph(srm_job *j)
+{
+    if (j->graph_exec) hipGraphExecDestroy(j->graph_exec);
+    if (j->graph) hipGraphDestroy(j->graph);
+    j->graph_exec = nullptr;
+    j->graph = nullptr;
+    j->graph_kernel = -1;
+}
+
+template <class T>
+int upload(srm_ctx *ctx, DevBuf &b, const T *src, size_t count)
+{
+    size_t bytes = count * sizeof(T);
+    int rc = ctx->ensure(b, std::max<size_t>(bytes, 64));
+    if (rc) return rc;
+    if (bytes) HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return SRM_OK;
+}
+
+// builds chunked matrix offsets for a plain index list
+void plan_plain(const std::vector<uint32_t> &idx, const srm_cand *cand, size_t budget_elems,
+                std::vector<uint64_t> &off, std::vector<PlainChunk> &chunks)
+{
+    off.resize(idx.size());
+    chunks.clear();
+    PlainChunk cur;
+    size_t acc = 0;
+    for (size_t j = 0; j < idx.size(); ++j) {
+        const srm_cand &c = cand[idx[j]];
+        size_t m = (size_t)(c.w + 1) * (size_t)(c.L + 1);
+        m = (m + 63) & ~(size_t)63;   // 128-byte aligned matrices
+        if (cur.count && acc + m > budget_elems) {
+            cur.mat_elems = acc;
+            chunks.push_back(cur);
+            cur = PlainChunk();
+            cur.begin = (uint32_t)j;
+            acc = 0;
+        }
+        off[j] = acc;
+        acc += m;
+        cur.count++;
+    }
+    if (cur.count) {
+        cur.mat_elems = acc;
+        chunks.push_back(cur);
+    }
+}
+
+int run_launches(srm_job *j, int kernel, bool timed)
+{
+    srm_ctx *ctx = j->ctx;
+    hipStream_t st = ctx->stream;
+    const bool use_band = kernel != SRM_KERNEL_PLAIN;
+    if (timed) HIP_TRY(hipEventRecord(j->ev[0], st));
+    if (use_band && !j->classes.empty()) {
+        HIP_TRY(srm::launch_reset((uint32_t *)j->d_queue.p, (int)j->classes.size(), st));
+        for (size_t c = 0; c < j->classes.size(); ++c) {
+            const BandClass &bc = j->classes[c];
+            srm::BandArgs a;
+            a.ma = j->ma;
+            a.mm = j->mm;
+            a.cand = (const srm_cand *)j->d_cand.p;
+            a.perm = (const uint32_t *)j->d_perm.p;
+            a.tiles = (const uint4 *)j->d_tiles.p + bc.tile_begin;
+            a.ntiles = bc.ntiles;
+            a.queue = (uint32_t *)j->d_queue.p + c;
+            a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
+            a.win_words = j->resident_genome
+                              ? (const uint32_t *)ctx->genome.p
+                              : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
+            a.scratch = (uint32_t *)j->d_scratch.p;
+            a.scratch_words_per_wave = (size_t)(bc.maxL + 1) * bc.nw * 64;
+            a.res = (srm_result *)j->d_res.p;
+            a.ops = (uint32_t *)j->d_ops.p;
+            HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, st));
+        }
+    }
+    if (timed) HIP_TRY(hipEventRecord(j->ev[1], st));
+    auto plain = [&](const std::vector<PlainChunk> &chunks, DevBuf &idx, DevBuf &off,
+                     bool codes) -> int {
+        for (const PlainChunk &ch : chunks) {
+            srm::PlainArgs a;
+            a.ma = j->ma;
+            a.mm = j->mm;
+            a.n = (int)ch.count;
+            a.idx = (const uint32_t *)idx.p + ch.begin;
+            a.cand = (const srm_cand *)j->d_cand.p;
+            a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
+            a.win_words = j->resident_genome
+                              ? (const uint32_t *)ctx->genome.p
+                              : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
+            a.read_bytes = (const char *)j->d_read_bytes.p;
+            a.win_bytes = (const char *)j->d_win_bytes.p;
+            a.mat = (int16_t *)j->d_mat.p;
+            a.mat_off = (const uint64_t *)off.p + ch.begin;
+            a.res = (srm_result *)j->d_res.p;
+            a.ops = (uint32_t *)j->d_ops.p;
+            HIP_TRY(srm::launch_plain(a, codes, st));
+        }
+        return SRM_OK;
+    };
+    int rc;
+    if (use_band)
+        rc = plain(j->chunks_rest, j->d_idx_rest, j->d_off_rest, true);
+    else
+        rc = plain(j->chunks_all, j->d_idx_all, j->d_off_all, true);
+    if (rc) return rc;
+    rc = plain(j->chunks_bytes, j->d_idx_bytes, j->d_off_bytes, false);
+    if (rc) return rc;
+    if (timed) HIP_TRY(hipEventRecord(j->ev[2], st));
+    return SRM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// extended API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *srm_version(void) { return "saruman-mi355x 0.1 (gfx950)"; }
+
+const char *srm_last_error(void) { return g_err.c_str(); }
+
+int srm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int srm_device_sync(int device)
+{
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceSynchronize());
+    return SRM_OK;
+}
+
+int srm_ctx_create(int device, size_t pool_cap_bytes, srm_ctx **out)
+{
+    *out = nullptr;
+    int n = srm_device_count();
+    if (n <= 0) return fail(SRM_ENODEV, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(SRM_EINVAL, "device %d out of range", device);
+    HIP_TRY(hipSetDevice(device));
+    srm_ctx *c = new srm_ctx();
+    c->device = device;
+    c->pool_cap = pool_cap_bytes;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return fail(SRM_EDEVICE, "hipGetDeviceProperties failed");
+    }
+    c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SRM_EDEVICE, "hipStreamCreate failed");
+    }
+    *out = c;
+    return SRM_OK;
+}
+
+void srm_job_destroy(srm_job *j);
+
+void srm_ctx_destroy(srm_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->dropin_job) srm_job_destroy(c->dropin_job);
+    hipStreamSynchronize(c->stream);
+    c->release(c->genome);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int srm_ctx_set_genome(srm_ctx *c, const uint32_t *words, size_t n_words)
+{
+    HIP_TRY(hipSetDevice(c->device));
+    int rc = c->ensure(c->genome, n_words * 4);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->genome.p, words, n_words * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->genome_words = n_words;
+    return SRM_OK;
+}
+
+int srm_job_create(srm_ctx *ctx, srm_job **out)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    srm_job *j = new srm_job();
+    j->ctx = ctx;
+    for (auto &e : j->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete j;
+            return fail(SRM_EDEVICE, "hipEventCreate failed");
+        }
+    }
+    *out = j;
+    return SRM_OK;
+}
+
+void srm_job_destroy(srm_job *j)
+{
+    if (!j) return;
+    srm_ctx *c = j->ctx;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    destroy_graph(j);
+    DevBuf *bufs[] = {&j->d_cand, &j->d_read_words, &j->d_win_words, &j->d_read_bytes,
+                      &j->d_win_bytes, &j->d_perm, &j->d_tiles, &j->d_queue, &j->d_scratch,
+                      &j->d_idx_rest, &j->d_idx_all, &j->d_idx_bytes, &j->d_off_rest,
+                      &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops};
+    for (DevBuf *b : bufs) c->release(*b);
+    for (auto &e : j->ev)
+        if (e) hipEventDestroy(e);
+    delete j;
+}
+
+int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, int n,
+                   const uint32_t *read_words, size_t n_read_words, const uint32_t *win_words,
+                   size_t n_win_words, const char *read_bytes, size_t n_read_bytes,
+                   const char *win_bytes, size_t n_win_bytes)
+{
+    srm_ctx *ctx = j->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    destroy_graph(j);
+    j->uploaded = false;
+    if (n < 0) return fail(SRM_EINVAL, "negative candidate count");
+    if (!win_words && !ctx->genome.p && n_win_bytes == 0)
+        return fail(SRM_EINVAL, "no window source: pass win_words or set a resident genome");
+    j->n = n;
+    j->ma = match;
+    j->mm = mismatch;
+    j->unit = (match == 0 && mismatch == 1);
+    j->resident_genome = (win_words == nullptr);
+    const bool band_ok = match >= 0 && mismatch >= 1;
+
+    // ---- validate and classify
+    std::vector<uint32_t> band_idx, rest_idx, bytes_idx;
+    size_t ops_end = 0;
+    const uint64_t read_codes = (uint64_t)n_read_words * 16;
+    const uint64_t win_codes =
+        j->resident_genome ? (uint64_t)ctx->genome_words * 16 : (uint64_t)n_win_words * 16;
+    const uint64_t pad = SRM_CODE_PAD;
+    for (int t = 0; t < n; ++t) {
+        const srm_cand &c = cand[t];
+        if (c.L < 1 || c.L > SRM_MAX_L || c.w > SRM_MAX_W)
+            return fail(SRM_EINVAL, "candidate %d: L=%d w=%d outside 1..%d / 0..%d", t, c.L, c.w,
+                        SRM_MAX_L, SRM_MAX_W);
+        size_t oe = (size_t)c.ops_off + srm_ops_words(c.w, c.k);
+        ops_end = std::max(ops_end, oe);
+        if (c.bits & SRM_CAND_BYTES) {
+            if ((uint64_t)c.read_off + c.L > n_read_bytes || (uint64_t)c.win_off + c.w > n_win_bytes)
+                return fail(SRM_EINVAL, "candidate %d: byte range out of bounds", t);
+            bytes_idx.push_back((uint32_t)t);
+            continue;
+        }
+        // code arrays: pad codes readable before 0 and after the end (the caller's
+        // arrays start with SRM_CODE_PAD pad codes for uploaded arrays)
+        if ((uint64_t)c.read_off + c.L + pad > read_codes - pad)
+            return fail(SRM_EINVAL, "candidate %d: read codes out of bounds", t);
+        if (j->resident_genome) {
+            if ((uint64_t)c.win_off < pad || (uint64_t)c.win_off + c.w + 2 * pad > win_codes)
+                return fail(SRM_EINVAL, "candidate %d: genome window out of bounds", t);
+        } else if ((uint64_t)c.win_off + c.w + pad > win_codes - pad) {
+            return fail(SRM_EINVAL, "candidate %d: window codes out of bounds", t);
+        }
+        if (band_ok && c.k <= SRM_BAND_KMAX && band_cells(c) <= SRM_BAND_MAX)
+            band_idx.push_back((uint32_t)t);
+        else
+            rest_idx.push_back((uint32_t)t);
+    }
+    j->n_band = band_idx.size();
+    j->n_rest = rest_idx.size();
+    j->n_bytes = bytes_idx.size();
+    j->ops_words = ops_end;
+
+    // ---- banded plan: sort by (class, L desc), 64-candidate tiles
+    std::vector<int> cls(n, 0);
+    for (uint32_t t : band_idx) cls[t] = (band_cells(cand[t]) + 7) / 8;
+    std::stable_sort(band_idx.begin(), band_idx.end(), [&](uint32_t a, uint32_t b) {
+        if (cls[a] != cls[b]) return cls[a] < cls[b];
+        return cand[a].L > cand[b].L;
+    });
+    std::vector<uint32_t> tiles;   // uint4 per tile
+    j->classes.clear();
+    size_t scratch_words = 0;
+    {
+        size_t i = 0;
+        while (i < band_idx.size()) {
+            BandClass bc;
+            bc.nw = cls[band_idx[i]];
+            bc.tile_begin = (uint32_t)(tiles.size() / 4);
+            bc.maxL = cand[band_idx[i]].L;
+            size_t e = i;
+            while (e < band_idx.size() && cls[band_idx[e]] == bc.nw) ++e;
+            for (size_t s = i; s < e; s += 64) {
+                uint32_t cnt = (uint32_t)std::min<size_t>(64, e - s);
+                tiles.push_back((uint32_t)s);
+                tiles.push_back(cnt);
+                tiles.push_back(cand[band_idx[s]].L);
+                tiles.push_back(0);
+            }
+            bc.ntiles = (uint32_t)(tiles.size() / 4) - bc.tile_begin;
+            // persistent grid: up to 2 blocks of 4 waves per CU, never more waves than tiles
+            int max_blocks = std::max(1, ctx->num_cus * 2);
+            bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
+            scratch_words = std::max(scratch_words, (size_t)bc.blocks * 4 * (size_t)(bc.maxL + 1) *
+                                                        bc.nw * 64);
+            j->classes.push_back(bc);
+            i = e;
+        }
+    }
+
+    // ---- plain plans
+    std::vector<uint32_t> all_idx(band_idx);
+    all_idx.insert(all_idx.end(), rest_idx.begin(), rest_idx.end());
+    std::sort(all_idx.begin(), all_idx.end());
+    std::vector<uint64_t> off_rest, off_all, off_bytes;
+    size_t free_b = 0, total_b = 0;
+    hipMemGetInfo(&free_b, &total_b);
+    size_t budget = std::min<size_t>(free_b / 4, (size_t)32 << 30);
+    if (ctx->pool_cap) budget = std::min(budget, ctx->pool_cap / 4);
+    budget = std::max<size_t>(budget, (size_t)64 << 20);
+    j->mat_budget_elems = budget / 2;
+    plan_plain(rest_idx, cand, j->mat_budget_elems, off_rest, j->chunks_rest);
+    plan_plain(all_idx, cand, j->mat_budget_elems, off_all, j->chunks_all);
+    plan_plain(bytes_idx, cand, j->mat_budget_elems, off_bytes, j->chunks_bytes);
+    size_t mat_elems = 0;
+    for (auto *v : {&j->chunks_rest, &j->chunks_all, &j->chunks_bytes})
+        for (auto &ch : *v) mat_elems = std::max(mat_elems, ch.mat_elems);
+
+    // ---- device buffers
+    int rc;
+    if ((rc = upload(ctx, j->d_cand, cand, (size_t)n))) return rc;
+    if ((rc = upload(ctx, j->d_read_words, read_words, n_read_words))) return rc;
+    if (!j->resident_genome && (rc = upload(ctx, j->d_win_words, win_words, n_win_words))) return rc;
+    if ((rc = upload(ctx, j->d_read_bytes, read_bytes, read_bytes ? n_read_bytes : 0))) return rc;
+    if ((rc = upload(ctx, j->d_win_bytes, win_bytes, win_bytes ? n_win_bytes : 0))) return rc;
+    if ((rc = upload(ctx, j->d_perm, band_idx.data(), band_idx.size()))) return rc;
+    if ((rc = upload(ctx, j->d_tiles, tiles.data(), tiles.size()))) return rc;
+    if ((rc = upload(ctx, j->d_idx_rest, rest_idx.data(), rest_idx.size()))) return rc;
+    if ((rc = upload(ctx, j->d_idx_all, all_idx.data(), all_idx.size()))) return rc;
+    if ((rc = upload(ctx, j->d_idx_bytes, bytes_idx.data(), bytes_idx.size()))) return rc;
+    if ((rc = upload(ctx, j->d_off_rest, off_rest.data(), off_rest.size()))) return rc;
+    if ((rc = upload(ctx, j->d_off_all, off_all.data(), off_all.size()))) return rc;
+    if ((rc = upload(ctx, j->d_off_bytes, off_bytes.data(), off_bytes.size()))) return rc;
+    if ((rc = ctx->ensure(j->d_queue, 64 * sizeof(uint32_t)))) return rc;
+    if ((rc = ctx->ensure(j->d_scratch, std::max<size_t>(scratch_words * 4, 256)))) return rc;
+    if ((rc = ctx->ensure(j->d_res, std::max<size_t>((size_t)n * sizeof(srm_result), 64)))) return rc;
+    if ((rc = ctx->ensure(j->d_ops, std::max<size_t>(ops_end * 4, 64)))) return rc;
+    if (mat_elems && (rc = ctx->ensure(j->d_mat, mat_elems * 2 + 256))) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    j->uploaded = true;
+    return SRM_OK;
+}
+
+int srm_job_run(srm_job *j, int kernel)
+{
+    if (!j->uploaded) return fail(SRM_ESTATE, "srm_job_run before a successful upload");
+    srm_ctx *ctx = j->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (j->graph_enabled) {
+        if (!j->graph_exec || j->graph_kernel != kernel) {
+            destroy_graph(j);
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = run_launches(j, kernel, false);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(SRM_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            j->graph = g;
+            HIP_TRY(hipGraphInstantiate(&j->graph_exec, g, nullptr, nullptr, 0));
+            j->graph_kernel = kernel;
+        }
+        HIP_TRY(hipEventRecord(j->ev[0], ctx->stream));
+        HIP_TRY(hipGraphLaunch(j->graph_exec, ctx->stream));
+        HIP_TRY(hipEventRecord(j->ev[2], ctx->stream));
+        HIP_TRY(hipEventSynchronize(j->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[2]));
+        j->band_ms = j->plain_ms = -1.f;
+        return SRM_OK;
+    }
+    int rc = run_launches(j, kernel, true);
+    if (rc) return rc;
+    HIP_TRY(hipEventSynchronize(j->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&j->band_ms, j->ev[0], j->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&j->plain_ms, j->ev[1], j->ev[2]));
+    return SRM_OK;
+}
+
+int srm_job_download(srm_job *j, srm_result *res, uint32_t *ops, size_t n_ops_words)
+{
+    srm_ctx *ctx = j->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (res && j->n)
+        HIP_TRY(hipMemcpyAsync(res, j->d_res.p, (size_t)j->n * sizeof(srm_result),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    if (ops && n_ops_words) {
+        size_t nw = std::min(n_ops_words, j->ops_words);
+        HIP_TRY(hipMemcpyAsync(ops, j->d_ops.p, nw * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SRM_OK;
+}
+
+float srm_job_last_ms(srm_job *j) { return j->last_ms; }
+
+float srm_job_kernel_ms(srm_job *j, int kind) { return kind == 0 ? j->band_ms : j->plain_ms; }
+
+int srm_job_use_graph(srm_job *j, int enable)
+{
+    j->graph_enabled = enable != 0;
+    if (!enable) destroy_graph(j);
+    return SRM_OK;
+}
+
+int srm_job_path_count(srm_job *j, int path)
+{
+    if (path == 0) return (int)j->n_band;
+    if (path == 1) return (int)j->n_rest;
+    return (int)j->n_bytes;
+}
+
+long long srm_pack_codes(const char *seq, size_t n, uint32_t *words, size_t code_off)
+{
+    long long bad = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t c;
+        switch (seq[i]) {
+        case 'A': c = 0; break;
+        case 'C': c = 1; break;
+        case 'G': c = 2; break;
+        case 'T': c = 3; break;
+        default: c = 0; ++bad; break;
+        }
+        size_t p = code_off + i;
+        uint32_t sh = (uint32_t)(p & 15) * 2;
+        words[p >> 4] = (words[p >> 4] & ~(3u << sh)) | (c << sh);
+    }
+    return bad;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// drop-in entry points
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_dropin_mu;
+srm_ctx *g_dropin_ctx[2] = {nullptr, nullptr};
+
+inline bool is_acgt(char c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
+
+int dropin_device()
+{
+    int n = srm_device_count();
+    if (n <= 0) return -1;
+    return use_gpu_value < n ? (int)use_gpu_value : 0;
+}
+
+srm_ctx *dropin_ctx(int which)
+{
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    if (!g_dropin_ctx[which]) {
+        int dev = dropin_device();
+        if (dev < 0) return nullptr;
+        srm_ctx *c = nullptr;
+        if (srm_ctx_create(dev, cuda_memory_per_thread, &c) != SRM_OK) return nullptr;
+        g_dropin_ctx[which] = c;
+    }
+    return g_dropin_ctx[which];
+}
+
+// id -> name map built by walking read_names in insertion order (util.h:62-66)
+void collect_names(std::unordered_map<int, const char *> &m)
+{
+    for (read_name_hash *e = read_names; e; e = (read_name_hash *)e->hh.next) m[e->read_number] = e->name;
+}
+}  // namespace
+
+extern "C" {
+
+void cuda_allocate_memory(int do_reverse)
+{
+    // cuda-functions.cu:541-562: do_reverse == 4 -> reverse pool, else forward
+    int which = do_reverse == 4 ? 1 : 0;
+    srm_ctx *c = dropin_ctx(which);
+    if (!c) {
+        fprintf(stderr, "ERROR: could not create GPU context: %s\n", srm_last_error());
+        exit(1);
+    }
+    // the anchor pointer the reference exposed; here a small allocation of the context
+    char **anchor = which ? &main_alloc_ptr_rev : &main_alloc_ptr_for;
+    if (!*anchor) {
+        hipSetDevice(c->device);
+        if (hipMalloc((void **)anchor, 256) != hipSuccess) *anchor = nullptr;
+    }
+}
+
+void cuda_free_memory(int do_reverse)
+{
+    // cuda-functions.cu:564-575: any non-zero do_reverse frees the reverse pool
+    int which = do_reverse ? 1 : 0;
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    char **anchor = which ? &main_alloc_ptr_rev : &main_alloc_ptr_for;
+    if (g_dropin_ctx[which]) {
+        hipSetDevice(g_dropin_ctx[which]->device);
+        if (*anchor) hipFree(*anchor);
+        srm_ctx_destroy(g_dropin_ctx[which]);
+        g_dropin_ctx[which] = nullptr;
+    }
+    *anchor = nullptr;
+}
+
+unsigned long int get_cuda_infos(void)
+{
+    // cuda-functions.cu:1051-1120
+    int count = srm_device_count();
+    fprintf(stderr, "\n=== CUDA hardware ===\n");
+    fprintf(stderr, "Detected %d GPU(s):\n\n", count);
+    size_t best_free = 0, sel_free = 0;
+    int best = 0;
+    for (int i = 0; i < count; ++i) {
+        size_t fr = 0, tot = 0;
+        hipDeviceProp_t prop;
+        hipSetDevice(i);
+        hipGetDeviceProperties(&prop, i);
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            fprintf(stderr, "Could not get memory information from GPU %d.\nShutting down.\n", i);
+            exit(-1);
+        }
+        if (fr > best_free) {
+            best_free = fr;
+            best = i;
+        }
+        if (i == (int)use_gpu_value) sel_free = fr;
+        fprintf(stderr, "GPU %d: %s\t[%*lu/%*lu MB available]\n", i, prop.name, 4,
+                (unsigned long)(fr / (1024 * 1024)), 4, (unsigned long)(tot / (1024 * 1024)));
+    }
+    if (count <= 0 || best_free < 128000000) {
+        fprintf(stderr,
+                "\n-> SARUMAN was not able to find a GPU with enough VRAM available! (available: %lu MB)\n",
+                (unsigned long)(best_free / (1024 * 1024)));
+        exit(1);
+    }
+    if ((int)use_gpu_value < count) {
+        fprintf(stderr, "\n-> SARUMAN was forced to run on GPU %d\n", use_gpu_value);
+        return (unsigned long)sel_free;   // the selected device's memory (Q22)
+    }
+    fprintf(stderr, "\n-> SARUMAN will run on GPU %d\n", best);
+    use_gpu_value = (unsigned short)best;
+    return (unsigned long)best_free;
+}
+
+void cuda_needleman_wunsch(int match, int mismatch, char *genome_sequences_on_host,
+                           char *read_sequences_on_host, int *genome_offsets, int *read_offsets,
+                           int *edit_offsets, int sample_size, int *read_id_array,
+                           int *genome_pos_array, char *direction_array, int error,
+                           size_t total_genome_array_size, size_t total_read_sequence_array_size,
+                           size_t total_edit_matrix_size, int do_reverse)
+{
+    (void)edit_offsets;
+    (void)total_edit_matrix_size;
+    fprintf(stderr, "\t\t-> starting CUDA kernel with %d samples from %s thread\n", sample_size,
+            do_reverse ? "reverse" : "forward");
+    if (do_reverse)
+        reverse_alignment_candidates += sample_size;   // cuda-functions.cu:604-608
+    else
+        forward_alignment_candidates += sample_size;
+    if (sample_size <= 0) return;
+    const int which = do_reverse ? 1 : 0;
+    srm_ctx *ctx = dropin_ctx(which);
+    if (!ctx) {
+        fprintf(stderr, "ERROR: no GPU context: %s\n", srm_last_error());
+        return;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const int n = sample_size, k = error;
+    if (k < 0 || k > 255) {
+        fprintf(stderr, "ERROR: error bound %d outside 0..255\n", k);
+        return;
+    }
+    const size_t win_total = total_genome_array_size - (size_t)sample_size * (size_t)error;
+    const size_t read_total = total_read_sequence_array_size;
+
+    // candidates in the A.1 layout (hash.c:1055-1059): exclusive end offsets
+    std::vector<srm_cand> cand((size_t)n);
+    std::vector<int> gstart((size_t)n), rstart((size_t)n);
+    bool any_bytes = false;
+    uint32_t ops_acc = 0;
+    const char *qenv = getenv("SARUMAN_QUIRK_FIRST_ROW0");
+    const bool quirk_first = qenv && qenv[0] == '1';
+    for (int t = 0; t < n; ++t) {
+        int gs = t ? genome_offsets[t - 1] : 0, rs = t ? read_offsets[t - 1] : 0;
+        int w = genome_offsets[t] - gs, L = read_offsets[t] - rs;
+        if (w < 0 || L < 1 || (size_t)gs + (size_t)w > win_total || (size_t)rs + (size_t)L > read_total ||
+            L > SRM_MAX_L || w > SRM_MAX_W) {
+            fprintf(stderr, "ERROR: malformed candidate %d (L=%d w=%d)\n", t, L, w);
+            return;
+        }
+        gstart[t] = gs;
+        rstart[t] = rs;
+        srm_cand &c = cand[t];
+        c.L = (uint16_t)L;
+        c.w = (uint16_t)w;
+        c.flag = (uint8_t)direction_array[t];
+        c.k = (uint8_t)k;
+        c.bits = (quirk_first && t == 0) ? SRM_CAND_STOP_ROW0 : 0;
+        c.pad = 0;
+        c.ops_off = ops_acc;
+        ops_acc += srm_ops_words((uint32_t)w, (uint32_t)k);
+        bool ok = true;
+        for (int i = 0; i < L && ok; ++i) ok = is_acgt(read_sequences_on_host[rs + i]);
+        for (int i = 0; i < w && ok; ++i) ok = is_acgt(genome_sequences_on_host[gs + i]);
+        // code index i of the kernel's view is packed byte i (the arrays below carry
+        // SRM_CODE_PAD codes of padding in front)
+        c.read_off = (uint32_t)rs;
+        c.win_off = (uint32_t)gs;
+        if (!ok) {
+            c.bits |= SRM_CAND_BYTES;
+            any_bytes = true;
+        }
+    }
+    // 2-bit codes with SRM_CODE_PAD codes of padding at both ends
+    std::vector<uint32_t> rw((read_total + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
+    std::vector<uint32_t> ww((win_total + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
+    srm_pack_codes(read_sequences_on_host, read_total, rw.data(), SRM_CODE_PAD);
+    srm_pack_codes(genome_sequences_on_host, win_total, ww.data(), SRM_CODE_PAD);
+
+    if (!ctx->dropin_job && srm_job_create(ctx, &ctx->dropin_job) != SRM_OK) {
+        fprintf(stderr, "ERROR: %s\n", srm_last_error());
+        return;
+    }
+    srm_job *job = ctx->dropin_job;
+    const char *kenv = getenv("SARUMAN_KERNEL");
+    int kernel = SRM_KERNEL_AUTO;
+    if (kenv && !strcmp(kenv, "plain")) kernel = SRM_KERNEL_PLAIN;
+    int rc = srm_job_upload(job, match, mismatch, cand.data(), n, rw.data(), rw.size(), ww.data(),
+                            ww.size(), any_bytes ? read_sequences_on_host : nullptr,
+                            any_bytes ? read_total : 0, any_bytes ? genome_sequences_on_host : nullptr,
+                            any_bytes ? win_total : 0);
+    if (rc == SRM_OK) rc = srm_job_run(job, kernel);
+    std::vector<srm_result> res((size_t)n);
+    std::vector<uint32_t> ops((size_t)ops_acc + 1);
+    if (rc == SRM_OK) rc = srm_job_download(job, res.data(), ops.data(), ops.size());
+    if (rc != SRM_OK) {
+        fprintf(stderr, "ERROR: GPU alignment failed: %s\n", srm_last_error());
+        return;
+    }
+    alignment_time[which] += srm_job_last_ms(job);
+
+    // host post-processing and records, cuda-functions.cu:828-1006
+    std::unordered_map<int, const char *> names;
+    collect_names(names);
+    std::string out;
+    std::vector<char> line(4096);
+    for (int t = 0; t < n; ++t) {
+        if (res[t].score > k) continue;
+        const srm_cand &c = cand[t];
+        auto it = names.find(read_id_array[t]);
+        const char *nm = it != names.end() ? it->second : "";
+        int len = srm_format_record(nm, (unsigned)genome_pos_array[t], c.flag, k, c.L, c.w, &res[t],
+                                    ops.data() + c.ops_off, read_sequences_on_host + rstart[t],
+                                    genome_sequences_on_host + gstart[t], line.data(), (int)line.size());
+        if (len >= (int)line.size()) {
+            line.resize((size_t)len + 1);
+            len = srm_format_record(nm, (unsigned)genome_pos_array[t], c.flag, k, c.L, c.w, &res[t],
+                                    ops.data() + c.ops_off, read_sequences_on_host + rstart[t],
+                                    genome_sequences_on_host + gstart[t], line.data(), (int)line.size());
+        }
+        if (len > 0) out.append(line.data(), (size_t)len);
+        if (c.flag > 3) {                               // :997-1003
+            if (reverse_output_control) reverse_output_control[read_id_array[t]] = 1;
+            reverse_alignments_done++;
+        } else {
+            if (forward_output_control) forward_output_control[read_id_array[t]] = 1;
+            forward_alignments_done++;
+        }
+    }
+    if (!out.empty()) {
+        fwrite(out.data(), 1, out.size(), stdout);
+        fflush(stdout);
+    }
+}
+
+}  // extern "C"
