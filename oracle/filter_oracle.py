@@ -27,6 +27,9 @@ from dataclasses import dataclass, field
 
 _COMP = {ord("A"): ord("T"), ord("C"): ord("G"), ord("G"): ord("C"), ord("T"): ord("A")}
 
+# how often the quirk paths fired (read by the tests to show they are exercised)
+STATS = {"spill": 0, "skip": 0, "leak": 0}
+
 
 def reverse_complement(seq: bytes) -> bytes:
     """util.c:35-57 -- A<->T, C<->G, every other byte unchanged, then reversed."""
@@ -150,7 +153,10 @@ def prepare_batch(strand: Strand, genome: bytes, reads: list, L: int, k: int, do
                     end_offset = e
                 p = ent.positions[i]
                 if p + L + start_offset + end_offset > total_bases:
+                    STATS["skip"] += 1
                     continue          # Q10: skips the reset below
+                if L + start_offset + end_offset != L + (s or 0) + (e or 0):
+                    STATS["leak"] += 1
                 w = L + start_offset + end_offset
                 win = genome[p:p + w]
                 win = win.split(b"\x00")[0]
@@ -207,6 +213,17 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
         def D(j):
             return data[j] if 0 <= j < len(data) else 0
 
+        def Db(list_start, idx, list_index):
+            # first-pass read of list b at index idx (Q12); counts reads that land
+            # outside list b
+            v = D(list_start + idx)
+            end = list_start
+            while end < len(data) and data[end] != 0:
+                end += 1
+            if list_start + idx > end:
+                STATS["spill"] += 1
+            return v
+
         # ---- run_seashore, hash.c:473-839 ----
         limit = read_number
         if (run + 1) * chunksize > limit:
@@ -240,7 +257,7 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
                         for b in range(a + 1, half):
                             rstart = b * q + 1
                             for err in range(-k, k + 1):
-                                Pb = D(offs[counter + b] + off)
+                                Pb = Db(offs[counter + b], off, b)
                                 nxt = 0
                                 while Pb > 0:
                                     if Pb == gs + rstart + err:

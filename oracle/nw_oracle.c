@@ -95,8 +95,10 @@ int oracle_fill(int ma, int mm, int k, const char *win, int w, const char *read,
 /*
  * Value-based traceback, cuda-functions.cu:108-315, in matrix-relative form.
  * Writes the emitted rows, already reversed (:306-307), into R/G (capacity cap,
- * caller zero-fills).  Returns the number of emitted columns, or -1 if the slot
- * capacity would be exceeded (never for mismatch >= 1).
+ * caller zero-fills).  Returns the number of emitted columns.  A walk longer than
+ * the slot (possible only with match < 0 or mismatch < 1) makes the reference
+ * write into the next candidate's slot; canonical (Q25): the walk stops when the
+ * slot is full.
  * end_row/end_col receive the cell where the walk stopped.
  */
 int oracle_traceback(int ma, int mm, int k, const char *win, int w, const char *read, int L,
@@ -137,8 +139,8 @@ int oracle_traceback(int ma, int mm, int k, const char *win, int w, const char *
         } else {
             break;                                     /* :262-276 */
         }
-        if (n >= cap)
-            return -1;
+        if (n >= cap)                                  /* Q25: slot full, stop */
+            break;
         R[n] = rc;
         G[n] = gc;
         ++n;

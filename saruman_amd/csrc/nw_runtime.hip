@@ -92,7 +92,7 @@ struct srm_ctx {
                             pool_cap, want);
         }
         if (b.p) {
-            hipFree(b.p);
+            (void)hipFree(b.p);
             pool_used -= b.cap;
             b.p = nullptr;
             b.cap = 0;
@@ -109,7 +109,7 @@ struct srm_ctx {
     void release(DevBuf &b)
     {
         if (b.p) {
-            hipFree(b.p);
+            (void)hipFree(b.p);
             pool_used -= b.cap;
         }
         b.p = nullptr;
@@ -172,8 +172,8 @@ inline int band_cells(const srm_cand &c)
 
 void destroy_graph(srm_job *j)
 {
-    if (j->graph_exec) hipGraphExecDestroy(j->graph_exec);
-    if (j->graph) hipGraphDestroy(j->graph);
+    if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
+    if (j->graph) (void)hipGraphDestroy(j->graph);
     j->graph_exec = nullptr;
     j->graph = nullptr;
     j->graph_kernel = -1;
@@ -183,7 +183,8 @@ template <class T>
 int upload(srm_ctx *ctx, DevBuf &b, const T *src, size_t count)
 {
     size_t bytes = count * sizeof(T);
-    int rc = ctx->ensure(b, std::max<size_t>(bytes, 64));
+    // 64 bytes of slack: a w == 0 window still reads its row-0 byte
+    int rc = ctx->ensure(b, bytes + 64);
     if (rc) return rc;
     if (bytes) HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     return SRM_OK;
@@ -337,11 +338,11 @@ void srm_job_destroy(srm_job *j);
 void srm_ctx_destroy(srm_ctx *c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     if (c->dropin_job) srm_job_destroy(c->dropin_job);
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     c->release(c->genome);
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -375,8 +376,8 @@ void srm_job_destroy(srm_job *j)
 {
     if (!j) return;
     srm_ctx *c = j->ctx;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     destroy_graph(j);
     DevBuf *bufs[] = {&j->d_cand, &j->d_read_words, &j->d_win_words, &j->d_read_bytes,
                       &j->d_win_bytes, &j->d_perm, &j->d_tiles, &j->d_queue, &j->d_scratch,
@@ -384,7 +385,7 @@ void srm_job_destroy(srm_job *j)
                       &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops};
     for (DevBuf *b : bufs) c->release(*b);
     for (auto &e : j->ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     delete j;
 }
 
@@ -438,7 +439,10 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         } else if ((uint64_t)c.win_off + c.w + pad > win_codes - pad) {
             return fail(SRM_EINVAL, "candidate %d: window codes out of bounds", t);
         }
-        if (band_ok && c.k <= SRM_BAND_KMAX && band_cells(c) <= SRM_BAND_MAX)
+        // banded: exact-band conditions, and w >= L - k (a shorter window can never
+        // hold a hit; the window code stream reads up to L + 93 codes past win_off)
+        if (band_ok && c.k <= SRM_BAND_KMAX && band_cells(c) <= SRM_BAND_MAX &&
+            (int)c.w >= (int)c.L - (int)c.k)
             band_idx.push_back((uint32_t)t);
         else
             rest_idx.push_back((uint32_t)t);
@@ -491,7 +495,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     std::sort(all_idx.begin(), all_idx.end());
     std::vector<uint64_t> off_rest, off_all, off_bytes;
     size_t free_b = 0, total_b = 0;
-    hipMemGetInfo(&free_b, &total_b);
+    (void)hipMemGetInfo(&free_b, &total_b);
     size_t budget = std::min<size_t>(free_b / 4, (size_t)32 << 30);
     if (ctx->pool_cap) budget = std::min(budget, ctx->pool_cap / 4);
     budget = std::max<size_t>(budget, (size_t)64 << 20);
@@ -541,7 +545,7 @@ int srm_job_run(srm_job *j, int kernel)
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (rc) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return rc;
             }
             if (e != hipSuccess) return fail(SRM_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
@@ -670,7 +674,7 @@ void cuda_allocate_memory(int do_reverse)
     // the anchor pointer the reference exposed; here a small allocation of the context
     char **anchor = which ? &main_alloc_ptr_rev : &main_alloc_ptr_for;
     if (!*anchor) {
-        hipSetDevice(c->device);
+        (void)hipSetDevice(c->device);
         if (hipMalloc((void **)anchor, 256) != hipSuccess) *anchor = nullptr;
     }
 }
@@ -682,8 +686,8 @@ void cuda_free_memory(int do_reverse)
     std::lock_guard<std::mutex> lk(g_dropin_mu);
     char **anchor = which ? &main_alloc_ptr_rev : &main_alloc_ptr_for;
     if (g_dropin_ctx[which]) {
-        hipSetDevice(g_dropin_ctx[which]->device);
-        if (*anchor) hipFree(*anchor);
+        (void)hipSetDevice(g_dropin_ctx[which]->device);
+        if (*anchor) (void)hipFree(*anchor);
         srm_ctx_destroy(g_dropin_ctx[which]);
         g_dropin_ctx[which] = nullptr;
     }
@@ -701,8 +705,8 @@ unsigned long int get_cuda_infos(void)
     for (int i = 0; i < count; ++i) {
         size_t fr = 0, tot = 0;
         hipDeviceProp_t prop;
-        hipSetDevice(i);
-        hipGetDeviceProperties(&prop, i);
+        (void)hipSetDevice(i);
+        (void)hipGetDeviceProperties(&prop, i);
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
             fprintf(stderr, "Could not get memory information from GPU %d.\nShutting down.\n", i);
             exit(-1);

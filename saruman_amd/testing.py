@@ -64,7 +64,7 @@ def random_batch(rng, n: int, L=36, k=2, flags=(1, 3, 5, 7), match=0, mismatch=1
             read = mutate(rng, src, nerr)
             if edge_cases and rng.random() < 0.05:
                 # leading insertions: read chars before the window start
-                read = np.concatenate([ACGT[rng.integers(0, 4, int(rng.integers(1, kt + 1)))],
+                read = np.concatenate([ACGT[rng.integers(0, 4, int(rng.integers(1, max(kt, 1) + 1)))],
                                        read])
             if len(read) < Lt:
                 read = np.concatenate([read, ACGT[rng.integers(0, 4, Lt - len(read))]])
