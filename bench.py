@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Benchmark: reads mapped/s of the batched persistent banded kernel vs the plain
+one-alignment-per-thread kernel, on one MI355X.
+
+A step = one pass of the alignment hot path (DP fill + traceback of every
+candidate of the workload, results left in HBM) over the whole candidate set of
+the workload, with the packed genome resident and the candidate batch already
+uploaded.  Default workload: the mixed-length one (4 Mbp genome with 2% repeat
+copies, 2,000,000 reads of 50-250 bp, k = ceil(5% L), both strands).
+
+Prints one JSON line (rank 0).  `value` = reads mapped per second of the batched
+kernel: distinct reads with at least one record (perfect hit from the filter or
+aligned hit with score <= k) divided by the step time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT32_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz, int32 VALU ops/s
+PEAK_HBM_GBS = 8000.0
+OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
+
+WORKLOAD_TEXT = {
+    "small": "1 Mbp uniform genome, 100,000 x 36 bp reads, 0-2 errors, k=2, both strands",
+    "single": "5 Mbp 50% GC genome, 1,000,000 x 100 bp reads (sub 1%, indel 0.1%, 5% random), "
+              "k=5, both strands",
+    "mixed": "4 Mbp genome with 200 repeat copies (2%, 0.5% divergence), 2,000,000 reads "
+             "50-250 bp (sub 1%, indel 0.1%), k=ceil(5% L), both strands",
+    "scale": "12 Mbp genome in 2 contigs (8+4 Mbp, 45% GC), 10,000,000 x 150 bp reads "
+             "(sub 1%, indel 0.2%), k=8, both strands",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def band_cells(L, k, flag):
+    """Algorithmic band cells: L(2k+1) for start-anchored flags, L(4k+1) for flags
+    with a free start (s = k)."""
+    s_free = np.isin(flag, (2, 3, 6, 7))
+    return np.where(s_free, L * (4 * k + 1), L * (2 * k + 1)).astype(np.float64)
+
+
+def hbm_bytes(L, w, k):
+    """Algorithmic HBM bytes per candidate: 2-bit read and window, 16 B descriptor,
+    2-bit transcript, 8 B result."""
+    return (np.ceil(L / 4) + np.ceil(w / 4) + 16 + np.ceil(2 * (L + 3 * k) / 8) + 8).astype(
+        np.float64)
+
+
+def cpu_baseline(genome, reads, db, n_sample, mapped_per_cand):
+    """The C oracle (full int16 matrix, scalar) over the first n_sample candidates,
+    OpenMP over all host cores."""
+    from oracle import oracle as orc
+    from saruman_amd.batch import Batch
+    from saruman_amd.workload import revcomp_all
+    n = min(n_sample, db.n)
+    g = genome.seq
+    fwd = reads.seq
+    rev = revcomp_all(reads).tobytes()
+    wins, rds = [], []
+    for t in range(n):
+        p, w = int(db.pos[t]), int(db.w[t])
+        r = int(db.read_id[t])
+        o, L = int(reads.off[r]), int(reads.length[r])
+        wins.append(g[p:p + w])
+        rds.append((rev if db.flag[t] > 3 else fwd)[o:o + L])
+    b = Batch(0, 1, int(db.k[:n].max()), wins, rds, db.flag[:n], db.pos[:n],
+              [f"r{int(x)}" for x in db.read_id[:n]], ks=db.k[:n])
+    res = orc.align_batch(b, with_records=False)
+    cand_rate = n / res.seconds
+    return dict(value=cand_rate * mapped_per_cand, unit="reads/s", cores=res.threads,
+                kind="port",
+                sample=f"oracle (full int16 matrix) over the first {n} candidates of the same "
+                       f"workload in {res.seconds:.2f} s on {res.threads} threads; "
+                       f"{cand_rate:.0f} candidates/s scaled by the workload's reads mapped "
+                       f"per candidate ({mapped_per_cand:.3f})",
+                candidates_per_s=cand_rate, seconds=res.seconds), res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="mixed", choices=sorted(WORKLOAD_TEXT))
+    ap.add_argument("--reads", type=int, default=None, help="override the read count")
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
+              "(candidates are independent; multi-GPU sharding is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import saruman_amd as sa
+    from saruman_amd import workload as wl
+
+    t0 = time.perf_counter()
+    cfg, genome, reads = wl.make_workload(args.workload, n_reads=args.reads)
+    t_gen = time.perf_counter() - t0
+    log(f"[bench] workload {args.workload}: genome {genome.n} bp, {reads.n} reads ({t_gen:.1f}s)")
+    t0 = time.perf_counter()
+    index = wl.Index(genome, wl.needed_qs(reads))
+    t_index = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    strands = [wl.run_filter(index, genome, reads, dr, cfg["chunk"]) for dr in (0, 4)]
+    t_filter = time.perf_counter() - t0
+    index.close()
+    db = wl.device_batch(reads, strands)
+    perfect_reads = np.unique(np.concatenate([s.perfect_read for s in strands]))
+    n_batches = sum(len(s.batch_end) for s in strands)
+    log(f"[bench] index {t_index:.1f}s, filter {t_filter:.1f}s: {db.n} candidates in "
+        f"{n_batches} reference batches, {len(perfect_reads)} reads with perfect hits")
+
+    ctx = sa.Context(0)
+    t0 = time.perf_counter()
+    ctx.set_genome(wl.genome_words(genome))
+    job = sa.Job(ctx)
+    job.upload(0, 1, db.cand, db.read_words, None, ops_total=db.ops_total)
+    t_upload = time.perf_counter() - t0
+    paths = job.paths()
+    log(f"[bench] uploaded in {t_upload:.2f}s; paths banded/plain/bytes = {paths}")
+    lib = sa.load()
+
+    # ---- batched persistent kernel: untimed warmup, then exactly K timed steps
+    if not args.no_graph:
+        job.use_graph(True)
+    for _ in range(args.warmup):
+        job.run("banded")
+    lib.srm_device_sync(0)
+    t0 = time.perf_counter()
+    dev_ms = []
+    for _ in range(args.steps):
+        dev_ms.append(job.run("banded"))
+    lib.srm_device_sync(0)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    # event-timed band kernels (non-graph launch, same stream)
+    job.use_graph(False)
+    job.run("banded")
+    band_ms, _ = job.kernel_ms()
+    t0 = time.perf_counter()
+    res, ops = job.download()
+    t_d2h = time.perf_counter() - t0
+    hit = res["score"].astype(np.int64) <= db.k
+    mapped = np.union1d(np.unique(db.read_id[hit]), perfect_reads)
+    n_mapped = len(mapped)
+    log(f"[bench] banded: {ms_per_step:.3f} ms/step (device {np.median(dev_ms):.3f} ms, "
+        f"band kernels {band_ms:.3f} ms), hits {int(hit.sum())}, reads mapped {n_mapped}")
+
+    # ---- plain kernel on the same job
+    plain = None
+    if not args.no_plain:
+        job.run("plain")   # one untimed pass (first-launch costs)
+        lib.srm_device_sync(0)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            job.run("plain")
+        lib.srm_device_sync(0)
+        p_ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+        pres, pops = job.download()
+        same = bool((np.minimum(pres["score"], db.k + 1) == np.minimum(res["score"], db.k + 1)).all()
+                    and (pres["n_ops"] == res["n_ops"]).all()
+                    and (pres["end_row"] == res["end_row"]).all()
+                    and (pres["end_col"] == res["end_col"]).all()
+                    and (pops == ops).all())
+        plain = dict(ms_per_step=p_ms, value=n_mapped / (p_ms / 1e3), results_identical=same)
+        log(f"[bench] plain: {p_ms:.1f} ms/step, identical results: {same}")
+    job.close()
+
+    L = db.L.astype(np.float64)
+    W = db.w.astype(np.float64)
+    K = db.k.astype(np.float64)
+    ref_cells = float((L * W).sum())
+    bcells = float(band_cells(L, K, db.flag).sum())
+    achieved = OPS_PER_BAND_CELL * bcells / (band_ms / 1e3) / 1e12
+    hbm = float(hbm_bytes(L, W, K).sum())
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, _ = cpu_baseline(genome, reads, db, args.cpu_sample, n_mapped / max(db.n, 1))
+        log(f"[bench] cpu oracle: {cpu['candidates_per_s']:.0f} candidates/s on {cpu['cores']} threads")
+
+    value = n_mapped / (ms_per_step / 1e3)
+    out = {
+        "metric": "reads mapped/sec (batched persistent banded kernel; GCUPS and the plain "
+                  "kernel alongside)",
+        "value": value,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}",
+                   "reads": int(reads.n), "candidates": int(db.n), "reads_mapped": int(n_mapped),
+                   "hits": int(hit.sum()), "match": 0, "mismatch": 1, "parallelism": "1 GPU",
+                   "graph": not args.no_graph},
+        "gcups": ref_cells / (ms_per_step / 1e3) / 1e9,
+        "band_gcups": bcells / (band_ms / 1e3) / 1e9,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
+                     "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS, "traffic": None,
+                     "kernel": "band_kernel (all band classes of one step)",
+                     "ops_per_step": OPS_PER_BAND_CELL * bcells,
+                     "kernel_ms": band_ms},
+        "roofline_hbm": {"achieved": hbm / (band_ms / 1e3) / 1e9, "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "algorithmic_bytes": hbm},
+        "plain": plain,
+        "speedup_vs_plain": (plain["ms_per_step"] / ms_per_step) if plain else None,
+        "cpu_baseline": cpu,
+        "host": {"generate_s": t_gen, "index_s": t_index, "filter_s": t_filter,
+                 "upload_s": t_upload, "download_s": t_d2h},
+        "e2e_reads_per_s_pcie": n_mapped / (t_upload + ms_per_step / 1e3 + t_d2h),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -200,6 +200,9 @@ void srm_ctx_destroy(srm_ctx *ctx);
  * windows from it (srm_cand.win_off = code index into it).
  */
 int srm_ctx_set_genome(srm_ctx *ctx, const uint32_t *words, size_t n_words);
+/* use the resident genome of another context on the same device (not copied; the
+ * source must outlive dst's jobs) */
+int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src);
 
 /*
  * A job: one batch of candidates, uploaded once, runnable many times.
@@ -242,6 +245,98 @@ int srm_slot_rows(const srm_result *res, const uint32_t *ops, const char *read, 
 /* pack ASCII ACGT into 2-bit codes; returns the number of non-ACGT bytes seen
  * (those are packed as A) */
 long long srm_pack_codes(const char *seq, size_t n, uint32_t *words, size_t code_off);
+
+/*
+ * Records of every hit of n candidates in candidate order (bulk form of
+ * srm_format_record): windows from the ASCII genome at pos[t], reads from
+ * reads_fwd / reads_rev (flags 0..3 / 4..7) at read_off[t], names name_prefix +
+ * read_id[t].  Returns the bytes needed (text complete only if <= outcap) or -1.
+ */
+long long srm_format_records(int n, const srm_result *res, const uint32_t *ops,
+                             const srm_cand *cand, const char *genome, const int64_t *pos,
+                             const char *reads_fwd, const char *reads_rev,
+                             const uint64_t *read_off, const int32_t *read_id,
+                             const char *name_prefix, char *out, size_t outcap,
+                             long long *n_hits, int threads);
+
+/* ===================================================================== */
+/* Part 3. Host candidate generation (reference semantics: src/hash.c,    */
+/*         src/saruman.c, src/util.c)                                     */
+/* ===================================================================== */
+
+typedef struct srm_index srm_index;
+typedef struct srm_filter srm_filter;
+
+/* q, rest, number of q-grams and exact_tresh for a read length and error bound
+ * (src/saruman.c:42-75, :119) */
+int srm_filter_params(int L, int k, int *q, int *rest, int *n, int *exact_tresh);
+
+/* q-gram index over every genome offset 0..len-q for each q in qs
+ * (replaces construct_qgram_index, src/util.c:456-499) */
+int srm_index_build(const char *genome, size_t len, const int *qs, int nq, int threads,
+                    srm_index **out);
+void srm_index_destroy(srm_index *index);
+
+/*
+ * Candidates of one strand (do_reverse 0 or 4) in the reference's batch order
+ * (replaces qgram_to_read_mapping/run_seashore/add_start_position/
+ * cuda_queue_check/prepare_cuda_alignments, src/hash.c:40-1111).  Each read has
+ * its own length and error bound.  chunk_value is the reference's -s.  blocked
+ * (may be NULL) marks separator bases between contigs.
+ */
+int srm_filter_run(const srm_index *index, const char *genome, size_t glen,
+                   const uint8_t *blocked, const char *reads, const uint64_t *read_off,
+                   const uint16_t *read_len, const uint8_t *read_k, size_t n_reads,
+                   int do_reverse, int chunk_value, int threads, srm_filter **out);
+int srm_filter_sizes(const srm_filter *f, uint64_t *n_cand, uint64_t *n_batches,
+                     uint64_t *n_perfect);
+int srm_filter_copy(const srm_filter *f, int32_t *cand_read, int64_t *cand_pos,
+                    uint8_t *cand_flag, uint16_t *cand_w, uint64_t *batch_end,
+                    int32_t *perfect_read, int64_t *perfect_pos);
+void srm_filter_destroy(srm_filter *f);
+
+/* seeded synthetic workloads (xoshiro256**) */
+int srm_synth_genome(uint64_t seed, size_t len, double gc, char *out);
+long long srm_synth_repeats(uint64_t seed, char *genome, size_t len, int n_families, int fam_min,
+                            int fam_max, int n_copies, double cover_frac, double divergence);
+int srm_synth_reads(uint64_t seed, const char *genome, const uint64_t *contig_lo,
+                    const uint64_t *contig_hi, int n_contigs, size_t n_reads, int Lmin, int Lmax,
+                    int mode, int max_err, double p_sub, double p_ins, int indel_max,
+                    double p_random, char *out, uint64_t *off, uint16_t *len, int64_t *true_pos,
+                    uint8_t *strand);
+
+/* ===================================================================== */
+/* Part 4. Serving: continuous batching over a resident genome            */
+/* ===================================================================== */
+
+typedef struct srm_server srm_server;
+
+typedef struct srm_server_config {
+    int device;
+    int n_slots;          /* requests in flight on the device at once (1..16) */
+    size_t hbm_budget;    /* device bytes the in-flight requests may use (0 = 32 GiB) */
+    int filter_threads;   /* host threads of the q-gram filter (0 = all) */
+    int format_threads;   /* host threads of record formatting (0 = all) */
+    int chunk_value;      /* the reference's -s for every request */
+    int kernel;           /* SRM_KERNEL_* */
+} srm_server_config;
+
+typedef struct srm_request_stats {
+    double t_submit, t_filtered, t_admitted, t_aligned, t_done;   /* s since server start */
+    double device_ms;
+    long long n_reads, n_candidates, n_hits, n_perfect, n_mapped;
+    int slot, status;
+} srm_request_stats;
+
+int srm_server_create(const srm_server_config *cfg, const char *genome, size_t glen,
+                      const int *qs, int nq, srm_server **out);
+/* returns the request id (> 0) or an SRM_E* code */
+long long srm_server_submit(srm_server *s, const char *reads, const uint64_t *read_off,
+                            const uint16_t *read_len, const uint8_t *read_k, size_t n_reads);
+int srm_server_wait(srm_server *s, long long id, int timeout_ms);
+long long srm_server_result(srm_server *s, long long id, char *out, size_t outcap,
+                            srm_request_stats *stats, int release);
+void srm_server_destroy(srm_server *s);
 
 #ifdef __cplusplus
 }

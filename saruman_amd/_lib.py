@@ -53,6 +53,31 @@ _PROTOS = [
                                     _vp, _vp, _vp, _vp, C.c_char_p, C.c_int]),
     ("srm_slot_rows", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     ("srm_pack_codes", C.c_longlong, [_vp, C.c_size_t, _vp, C.c_size_t]),
+    ("srm_ctx_borrow_genome", C.c_int, [_vp, _vp]),
+    ("srm_format_records", C.c_longlong, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          C.c_char_p, _vp, C.c_size_t, C.POINTER(C.c_longlong),
+                                          C.c_int]),
+    # host candidate generation and synthetic workloads
+    ("srm_filter_params", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    ("srm_index_build", C.c_int, [_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("srm_index_destroy", None, [_vp]),
+    ("srm_filter_run", C.c_int, [_vp, _vp, C.c_size_t, _vp, _vp, _vp, _vp, _vp, C.c_size_t,
+                                 C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("srm_filter_sizes", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("srm_filter_copy", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("srm_filter_destroy", None, [_vp]),
+    ("srm_synth_genome", C.c_int, [C.c_uint64, C.c_size_t, C.c_double, _vp]),
+    ("srm_synth_repeats", C.c_longlong, [C.c_uint64, _vp, C.c_size_t, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_double, C.c_double]),
+    ("srm_synth_reads", C.c_int, [C.c_uint64, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_int,
+                                  C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    # serving
+    ("srm_server_create", C.c_int, [_vp, _vp, C.c_size_t, _vp, C.c_int, C.POINTER(_vp)]),
+    ("srm_server_submit", C.c_longlong, [_vp, _vp, _vp, _vp, _vp, C.c_size_t]),
+    ("srm_server_wait", C.c_int, [_vp, C.c_longlong, C.c_int]),
+    ("srm_server_result", C.c_longlong, [_vp, C.c_longlong, _vp, C.c_size_t, _vp, C.c_int]),
+    ("srm_server_destroy", None, [_vp]),
     # drop-in entry points (src/cuda-functions.h:55-78)
     ("cuda_needleman_wunsch", None, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp,
                                      _vp, _vp, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t,

@@ -49,5 +49,6 @@ struct BandArgs {
 hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st);
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
+int band_occupancy(int nw, bool unit);
 
 }  // namespace srm

@@ -369,6 +369,33 @@ static hipError_t launch_band_nw(const BandArgs &a, bool unit, int blocks, hipSt
     return hipGetLastError();
 }
 
+template <int NW>
+static int occupancy_nw(bool unit)
+{
+    int blocks = 0;
+    hipError_t e = unit ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &blocks, (const void *)band_kernel<NW, true>, 256, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &blocks, (const void *)band_kernel<NW, false>, 256, 0);
+    return e == hipSuccess && blocks > 0 ? blocks : 1;
+}
+
+// resident 256-thread blocks per CU of band_kernel<nw, unit>
+int band_occupancy(int nw, bool unit)
+{
+    switch (nw) {
+    case 1: return occupancy_nw<1>(unit);
+    case 2: return occupancy_nw<2>(unit);
+    case 3: return occupancy_nw<3>(unit);
+    case 4: return occupancy_nw<4>(unit);
+    case 5: return occupancy_nw<5>(unit);
+    case 6: return occupancy_nw<6>(unit);
+    case 7: return occupancy_nw<7>(unit);
+    case 8: return occupancy_nw<8>(unit);
+    default: return 1;
+    }
+}
+
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st)
 {
     switch (nw) {

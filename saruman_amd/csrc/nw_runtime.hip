@@ -75,8 +75,13 @@ struct srm_ctx {
     size_t pool_cap = 0;      // 0 = unlimited
     size_t pool_used = 0;     // bytes held by this context's buffers
     int num_cus = 0;
-    DevBuf genome;            // resident packed genome
+    DevBuf genome;            // resident packed genome (owned)
+    const uint32_t *genome_borrowed = nullptr;   // or another context's (same device)
     size_t genome_words = 0;
+    const uint32_t *genome_ptr() const
+    {
+        return genome_borrowed ? genome_borrowed : (const uint32_t *)genome.p;
+    }
     std::mutex mu;
     srm_job *dropin_job = nullptr;
 
@@ -239,7 +244,7 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.queue = (uint32_t *)j->d_queue.p + c;
             a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
             a.win_words = j->resident_genome
-                              ? (const uint32_t *)ctx->genome.p
+                              ? ctx->genome_ptr()
                               : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
             a.scratch = (uint32_t *)j->d_scratch.p;
             a.scratch_words_per_wave = (size_t)(bc.maxL + 1) * bc.nw * 64;
@@ -260,7 +265,7 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.cand = (const srm_cand *)j->d_cand.p;
             a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
             a.win_words = j->resident_genome
-                              ? (const uint32_t *)ctx->genome.p
+                              ? ctx->genome_ptr()
                               : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
             a.read_bytes = (const char *)j->d_read_bytes.p;
             a.win_bytes = (const char *)j->d_win_bytes.p;
@@ -354,6 +359,17 @@ int srm_ctx_set_genome(srm_ctx *c, const uint32_t *words, size_t n_words)
     HIP_TRY(hipMemcpyAsync(c->genome.p, words, n_words * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->genome_words = n_words;
+    c->genome_borrowed = nullptr;
+    return SRM_OK;
+}
+
+int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src)
+{
+    if (!src->genome_ptr()) return fail(SRM_ESTATE, "source context has no genome");
+    if (dst->device != src->device) return fail(SRM_EINVAL, "contexts on different devices");
+    dst->release(dst->genome);
+    dst->genome_borrowed = src->genome_ptr();
+    dst->genome_words = src->genome_words;
     return SRM_OK;
 }
 
@@ -400,7 +416,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     destroy_graph(j);
     j->uploaded = false;
     if (n < 0) return fail(SRM_EINVAL, "negative candidate count");
-    if (!win_words && !ctx->genome.p && n_win_bytes == 0)
+    if (!win_words && !ctx->genome_ptr() && n_win_bytes == 0)
         return fail(SRM_EINVAL, "no window source: pass win_words or set a resident genome");
     j->n = n;
     j->ma = match;
@@ -434,7 +450,8 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         if ((uint64_t)c.read_off + c.L + pad > read_codes - pad)
             return fail(SRM_EINVAL, "candidate %d: read codes out of bounds", t);
         if (j->resident_genome) {
-            if ((uint64_t)c.win_off < pad || (uint64_t)c.win_off + c.w + 2 * pad > win_codes)
+            // the genome carries SRM_CODE_PAD codes of padding at both ends
+            if ((uint64_t)c.win_off < pad || (uint64_t)c.win_off + c.w + pad > win_codes)
                 return fail(SRM_EINVAL, "candidate %d: genome window out of bounds", t);
         } else if ((uint64_t)c.win_off + c.w + pad > win_codes - pad) {
             return fail(SRM_EINVAL, "candidate %d: window codes out of bounds", t);
@@ -479,8 +496,10 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
                 tiles.push_back(0);
             }
             bc.ntiles = (uint32_t)(tiles.size() / 4) - bc.tile_begin;
-            // persistent grid: up to 2 blocks of 4 waves per CU, never more waves than tiles
-            int max_blocks = std::max(1, ctx->num_cus * 2);
+            // persistent grid: as many 4-wave blocks as stay resident on every CU
+            // (occupancy of this kernel instance), never more waves than tiles
+            const int per_cu = srm::band_occupancy(bc.nw, match == 0 && mismatch == 1);
+            int max_blocks = std::max(1, ctx->num_cus * per_cu);
             bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
             scratch_words = std::max(scratch_words, (size_t)bc.blocks * 4 * (size_t)(bc.maxL + 1) *
                                                         bc.nw * 64);

@@ -4,6 +4,9 @@
 // record of the reference's post-processing (src/cuda-functions.cu:828-994).
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "saruman_hip.h"
@@ -107,4 +110,73 @@ extern "C" int srm_format_record(const char *name, unsigned int pos, int flag, i
     return std::snprintf(out, (size_t)(outcap > 0 ? outcap : 0), "%s\t%d\t%d\t%s\t%.*s\t%.*s\t%d\n",
                          name, (int)start, (int)stop, flag > 3 ? "<<" : ">>", rn, R + lead, gn,
                          G + lead, (int)res->score);
+}
+
+// ---------------------------------------------------------------------------
+// bulk formatting
+// ---------------------------------------------------------------------------
+
+/*
+ * Records of every hit (score <= k) of n candidates, in candidate order, into
+ * out (capacity outcap).  Windows are taken from the ASCII genome at pos[t];
+ * reads from reads_fwd (flags 0..3) or reads_rev (flags 4..7) at read_off[t];
+ * names are name_prefix followed by read_id[t].  Returns the number of bytes the
+ * records need (the text is complete only if that is <= outcap), or -1.
+ * n_hits receives the number of records.
+ */
+extern "C" long long srm_format_records(int n, const srm_result *res, const uint32_t *ops,
+                                        const srm_cand *cand, const char *genome,
+                                        const int64_t *pos, const char *reads_fwd,
+                                        const char *reads_rev, const uint64_t *read_off,
+                                        const int32_t *read_id, const char *name_prefix,
+                                        char *out, size_t outcap, long long *n_hits,
+                                        int threads)
+{
+    if (n < 0) return -1;
+    int T = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    T = std::max(1, std::min(T, 16));
+    std::vector<std::string> part((size_t)T);
+    std::vector<long long> hits((size_t)T, 0);
+    std::vector<int> bad((size_t)T, 0);
+    const size_t per = ((size_t)n + (size_t)T - 1) / (size_t)T;
+    auto work = [&](int t) {
+        size_t b = (size_t)t * per, e = std::min((size_t)n, b + per);
+        char line[2 * (SRM_MAX_W + 256) + 256];
+        char nm[96];
+        std::string &dst = part[(size_t)t];
+        for (size_t i = b; i < e; ++i) {
+            const srm_cand &c = cand[i];
+            if (res[i].score > (int)c.k) continue;
+            snprintf(nm, sizeof nm, "%s%d", name_prefix, read_id[i]);
+            const char *rd = (c.flag > 3 ? reads_rev : reads_fwd) + read_off[i];
+            int len = srm_format_record(nm, (unsigned)pos[i], c.flag, c.k, c.L, c.w, &res[i],
+                                        ops + c.ops_off, rd, genome + pos[i], line,
+                                        (int)sizeof line);
+            if (len < 0 || len >= (int)sizeof line) {
+                bad[(size_t)t] = 1;
+                continue;
+            }
+            dst.append(line, (size_t)len);
+            hits[(size_t)t]++;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    long long total = 0, nh = 0;
+    for (int t = 0; t < T; ++t) {
+        if (bad[(size_t)t]) return -1;
+        total += (long long)part[(size_t)t].size();
+        nh += hits[(size_t)t];
+    }
+    if (n_hits) *n_hits = nh;
+    if ((size_t)total <= outcap) {
+        size_t o = 0;
+        for (auto &p : part) {
+            std::memcpy(out + o, p.data(), p.size());
+            o += p.size();
+        }
+    }
+    return total;
 }

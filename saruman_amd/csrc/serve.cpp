@@ -1,0 +1,438 @@
+// serve.cpp -- continuous-batching request loop around the alignment path.
+//
+// A server keeps the packed genome resident on one GPU and runs requests (read
+// batches of any size) through three overlapping stages:
+//   filter   (host thread)   q-gram filter of both strands -> candidates, codes
+//   admit    (host thread)   a request enters as soon as one of n_slots slots is
+//                            free and its device footprint fits the HBM budget
+//   slot k   (host thread)   upload, persistent banded kernel, download and record
+//                            formatting on the slot's own HIP stream
+// so the filtering of later requests, the device work of several requests and
+// the formatting of finished ones run at the same time.  A request's records do
+// not depend on what runs beside it: its candidates, kernels and output order are
+// a function of the request alone.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "saruman_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+inline char comp(char c)
+{
+    switch (c) {
+    case 'A': return 'T';
+    case 'C': return 'G';
+    case 'G': return 'C';
+    case 'T': return 'A';
+    default: return c;
+    }
+}
+
+struct Request {
+    uint64_t id = 0;
+    std::string reads;
+    std::vector<uint64_t> off;
+    std::vector<uint16_t> len;
+    std::vector<uint8_t> k;
+    // after the filter
+    std::string reads_rev;
+    std::vector<srm_cand> cand;
+    std::vector<int64_t> pos;
+    std::vector<int32_t> rid;
+    std::vector<uint64_t> roff_bytes;
+    std::vector<uint32_t> read_words;
+    std::vector<int32_t> perf_read[2];
+    std::vector<int64_t> perf_pos[2];
+    size_t ops_total = 0;
+    size_t dev_bytes = 0;
+    // results
+    std::string records;
+    srm_request_stats st{};
+    bool done = false;
+};
+
+}  // namespace
+
+struct srm_server {
+    srm_server_config cfg{};
+    std::string genome;               // ASCII, for records
+    srm_index *index = nullptr;
+    std::vector<srm_ctx *> ctx;       // one per slot; slot 0 owns the resident genome
+    std::vector<srm_job *> job;
+    Clock::time_point t0;
+
+    std::mutex mu;
+    std::condition_variable cv;       // any state change
+    std::deque<std::shared_ptr<Request>> incoming, ready;
+    std::map<uint64_t, std::shared_ptr<Request>> all;
+    std::vector<std::shared_ptr<Request>> slot_req;
+    std::vector<bool> slot_busy;
+    size_t budget_used = 0;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::vector<std::thread> threads;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+    void filter_loop();
+    void admit_loop();
+    void slot_loop(int s);
+    void prepare(Request &r);
+    void run_on_slot(int s, Request &r);
+};
+
+void srm_server::prepare(Request &r)
+{
+    const size_t n = r.len.size();
+    const size_t total = r.reads.size();
+    r.reads_rev.resize(total);
+    for (size_t i = 0; i < n; ++i) {
+        const char *src = r.reads.data() + r.off[i];
+        char *dst = &r.reads_rev[r.off[i]];
+        for (int j = 0; j < r.len[i]; ++j) dst[j] = comp(src[r.len[i] - 1 - j]);
+    }
+    // both strands through the reference filter semantics
+    for (int s = 0; s < 2; ++s) {
+        srm_filter *F = nullptr;
+        int rc = srm_filter_run(index, genome.data(), genome.size(), nullptr, r.reads.data(),
+                                r.off.data(), r.len.data(), r.k.data(), n, s ? 4 : 0,
+                                cfg.chunk_value, cfg.filter_threads, &F);
+        if (rc != SRM_OK) {
+            r.st.status = rc;
+            return;
+        }
+        uint64_t nc, nb, np;
+        srm_filter_sizes(F, &nc, &nb, &np);
+        std::vector<int32_t> cr(nc);
+        std::vector<int64_t> cp(nc);
+        std::vector<uint8_t> cf(nc);
+        std::vector<uint16_t> cw(nc);
+        r.perf_read[s].resize(np);
+        r.perf_pos[s].resize(np);
+        srm_filter_copy(F, cr.data(), cp.data(), cf.data(), cw.data(), nullptr,
+                        r.perf_read[s].data(), r.perf_pos[s].data());
+        srm_filter_destroy(F);
+        for (size_t t = 0; t < nc; ++t) {
+            const int32_t i = cr[t];
+            srm_cand c;
+            c.read_off = (uint32_t)(r.off[(size_t)i] + (cf[t] > 3 ? total : 0));
+            c.win_off = (uint32_t)(cp[t] + SRM_CODE_PAD);
+            c.ops_off = (uint32_t)r.ops_total;
+            c.L = r.len[(size_t)i];
+            c.w = cw[t];
+            c.flag = cf[t];
+            c.k = r.k[(size_t)i];
+            c.bits = 0;
+            c.pad = 0;
+            r.ops_total += srm_ops_words(c.w, c.k);
+            r.cand.push_back(c);
+            r.pos.push_back(cp[t]);
+            r.rid.push_back(i);
+            r.roff_bytes.push_back(r.off[(size_t)i]);
+        }
+    }
+    // read codes: forward reads, then their reverse complements
+    r.read_words.assign((2 * total + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
+    srm_pack_codes(r.reads.data(), total, r.read_words.data(), SRM_CODE_PAD);
+    srm_pack_codes(r.reads_rev.data(), total, r.read_words.data(), SRM_CODE_PAD + total);
+    // device footprint: inputs, results, transcripts, tiles, banded scratch bound
+    size_t maxL = 0;
+    for (auto L : r.len) maxL = std::max<size_t>(maxL, L);
+    const size_t nc = r.cand.size();
+    const size_t waves = std::min<size_t>((nc + 63) / 64, 256 * 16);
+    r.dev_bytes = r.read_words.size() * 4 + nc * (sizeof(srm_cand) + sizeof(srm_result) + 4 * 4) +
+                  r.ops_total * 4 + waves * (maxL + 1) * 8 * 64 * 4 + (1u << 20);
+    r.st.n_reads = (long long)n;
+    r.st.n_candidates = (long long)nc;
+}
+
+void srm_server::run_on_slot(int s, Request &r)
+{
+    srm_job *j = job[(size_t)s];
+    const int n = (int)r.cand.size();
+    int rc = SRM_OK;
+    std::vector<srm_result> res((size_t)n);
+    std::vector<uint32_t> ops(r.ops_total + 1);
+    if (n > 0) {
+        rc = srm_job_upload(j, 0, 1, r.cand.data(), n, r.read_words.data(), r.read_words.size(),
+                            nullptr, 0, nullptr, 0, nullptr, 0);
+        if (rc == SRM_OK) rc = srm_job_run(j, cfg.kernel);
+        if (rc == SRM_OK) r.st.device_ms = srm_job_last_ms(j);
+        if (rc == SRM_OK) rc = srm_job_download(j, res.data(), ops.data(), ops.size());
+    }
+    r.st.t_aligned = now();
+    if (rc != SRM_OK) {
+        r.st.status = rc;
+        return;
+    }
+    // records: perfect hits (hash.c:580) of each strand, then aligned hits in
+    // candidate order
+    std::string out;
+    std::vector<char> mapped(r.len.size(), 0);
+    char line[2 * (SRM_MAX_L + 64) + 128];
+    for (int sd = 0; sd < 2; ++sd) {
+        for (size_t p = 0; p < r.perf_read[sd].size(); ++p) {
+            const int32_t i = r.perf_read[sd][p];
+            const int64_t S = r.perf_pos[sd][p];
+            const int L = r.len[(size_t)i];
+            const char *rd = (sd ? r.reads_rev.data() : r.reads.data()) + r.off[(size_t)i];
+            int len = snprintf(line, sizeof line, "r%d\t%d\t%d\t%s\t%.*s\t%.*s\t%d\n", i, (int)S,
+                               (int)(S + L - 1), sd ? "<<" : ">>", L, genome.data() + S, L, rd, 0);
+            if (len > 0 && len < (int)sizeof line) out.append(line, (size_t)len);
+            mapped[(size_t)i] = 1;
+        }
+    }
+    r.st.n_perfect = (long long)(r.perf_read[0].size() + r.perf_read[1].size());
+    if (n > 0) {
+        long long hits = 0;
+        long long need = srm_format_records(n, res.data(), ops.data(), r.cand.data(), genome.data(),
+                                            r.pos.data(), r.reads.data(), r.reads_rev.data(),
+                                            r.roff_bytes.data(), r.rid.data(), "r", nullptr, 0,
+                                            &hits, cfg.format_threads);
+        if (need < 0) {
+            r.st.status = SRM_EINVAL;
+            return;
+        }
+        size_t base = out.size();
+        out.resize(base + (size_t)need);
+        srm_format_records(n, res.data(), ops.data(), r.cand.data(), genome.data(), r.pos.data(),
+                           r.reads.data(), r.reads_rev.data(), r.roff_bytes.data(), r.rid.data(),
+                           "r", &out[base], (size_t)need, &hits, cfg.format_threads);
+        r.st.n_hits = hits;
+        for (int t = 0; t < n; ++t)
+            if (res[(size_t)t].score <= (int)r.cand[(size_t)t].k) mapped[(size_t)r.rid[(size_t)t]] = 1;
+    }
+    long long m = 0;
+    for (char x : mapped) m += x;
+    r.st.n_mapped = m;
+    r.records.swap(out);
+}
+
+void srm_server::filter_loop()
+{
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stop || !incoming.empty(); });
+            if (stop) return;
+            r = incoming.front();
+            incoming.pop_front();
+        }
+        prepare(*r);
+        r->st.t_filtered = now();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (r->st.status != SRM_OK) {
+                r->done = true;
+                r->st.t_done = now();
+            } else if (r->dev_bytes > cfg.hbm_budget) {
+                r->st.status = SRM_ENOMEM;   // can never be admitted
+                r->done = true;
+                r->st.t_done = now();
+            } else {
+                ready.push_back(r);
+            }
+        }
+        cv.notify_all();
+    }
+}
+
+void srm_server::admit_loop()
+{
+    for (;;) {
+        std::unique_lock<std::mutex> lk(mu);
+        int slot = -1;
+        cv.wait(lk, [&] {
+            if (stop) return true;
+            if (ready.empty()) return false;
+            if (budget_used + ready.front()->dev_bytes > cfg.hbm_budget) return false;
+            for (size_t s = 0; s < slot_busy.size(); ++s)
+                if (!slot_busy[s]) {
+                    slot = (int)s;
+                    return true;
+                }
+            return false;
+        });
+        if (stop) return;
+        auto r = ready.front();
+        ready.pop_front();
+        budget_used += r->dev_bytes;
+        slot_busy[(size_t)slot] = true;
+        slot_req[(size_t)slot] = r;
+        r->st.slot = slot;
+        r->st.t_admitted = now();
+        lk.unlock();
+        cv.notify_all();
+    }
+}
+
+void srm_server::slot_loop(int s)
+{
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stop || slot_req[(size_t)s] != nullptr; });
+            if (stop) return;
+            r = slot_req[(size_t)s];
+        }
+        run_on_slot(s, *r);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            r->st.t_done = now();
+            r->done = true;
+            budget_used -= r->dev_bytes;
+            slot_req[(size_t)s] = nullptr;
+            slot_busy[(size_t)s] = false;
+            // inputs are no longer needed
+            std::vector<srm_cand>().swap(r->cand);
+            std::vector<uint32_t>().swap(r->read_words);
+        }
+        cv.notify_all();
+    }
+}
+
+extern "C" {
+
+int srm_server_create(const srm_server_config *cfg, const char *genome, size_t glen,
+                      const int *qs, int nq, srm_server **out)
+{
+    *out = nullptr;
+    if (cfg->n_slots < 1 || cfg->n_slots > 16 || cfg->chunk_value < 1) return SRM_EINVAL;
+    auto *S = new srm_server();
+    S->cfg = *cfg;
+    if (S->cfg.hbm_budget == 0) S->cfg.hbm_budget = (size_t)32 << 30;
+    if (S->cfg.kernel < 0) S->cfg.kernel = SRM_KERNEL_AUTO;
+    S->t0 = Clock::now();
+    S->genome.assign(genome, glen);
+    int rc = srm_index_build(genome, glen, qs, nq, cfg->filter_threads, &S->index);
+    if (rc != SRM_OK) {
+        delete S;
+        return rc;
+    }
+    std::vector<uint32_t> gw((glen + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
+    srm_pack_codes(genome, glen, gw.data(), SRM_CODE_PAD);
+    for (int s = 0; s < cfg->n_slots; ++s) {
+        srm_ctx *c = nullptr;
+        rc = srm_ctx_create(cfg->device, 0, &c);
+        if (rc == SRM_OK) rc = s == 0 ? srm_ctx_set_genome(c, gw.data(), gw.size())
+                                      : srm_ctx_borrow_genome(c, S->ctx[0]);
+        srm_job *j = nullptr;
+        if (rc == SRM_OK) rc = srm_job_create(c, &j);
+        if (rc != SRM_OK) {
+            if (c) srm_ctx_destroy(c);
+            for (size_t i = S->ctx.size(); i-- > 0;) {
+                srm_job_destroy(S->job[i]);
+                srm_ctx_destroy(S->ctx[i]);
+            }
+            srm_index_destroy(S->index);
+            delete S;
+            return rc;
+        }
+        S->ctx.push_back(c);
+        S->job.push_back(j);
+    }
+    S->slot_req.assign((size_t)cfg->n_slots, nullptr);
+    S->slot_busy.assign((size_t)cfg->n_slots, false);
+    S->threads.emplace_back([S] { S->filter_loop(); });
+    S->threads.emplace_back([S] { S->admit_loop(); });
+    for (int s = 0; s < cfg->n_slots; ++s) S->threads.emplace_back([S, s] { S->slot_loop(s); });
+    *out = S;
+    return SRM_OK;
+}
+
+long long srm_server_submit(srm_server *S, const char *reads, const uint64_t *read_off,
+                            const uint16_t *read_len, const uint8_t *read_k, size_t n_reads)
+{
+    auto r = std::make_shared<Request>();
+    uint64_t total = 0;
+    for (size_t i = 0; i < n_reads; ++i) {
+        if (read_len[i] < 1 || read_len[i] > SRM_MAX_L) return SRM_EINVAL;
+        total = std::max<uint64_t>(total, read_off[i] + read_len[i]);
+    }
+    // repack the reads contiguously
+    r->off.resize(n_reads);
+    r->len.assign(read_len, read_len + n_reads);
+    r->k.assign(read_k, read_k + n_reads);
+    size_t acc = 0;
+    for (size_t i = 0; i < n_reads; ++i) {
+        r->off[i] = acc;
+        acc += read_len[i];
+    }
+    r->reads.resize(acc);
+    for (size_t i = 0; i < n_reads; ++i)
+        std::memcpy(&r->reads[r->off[i]], reads + read_off[i], read_len[i]);
+    std::lock_guard<std::mutex> lk(S->mu);
+    r->id = S->next_id++;
+    r->st.t_submit = S->now();
+    r->st.slot = -1;
+    S->all[r->id] = r;
+    S->incoming.push_back(r);
+    S->cv.notify_all();
+    return (long long)r->id;
+}
+
+/* waits up to timeout_ms (< 0: forever); 1 = done, 0 = not yet, < 0 = unknown id */
+int srm_server_wait(srm_server *S, long long id, int timeout_ms)
+{
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find((uint64_t)id);
+    if (it == S->all.end()) return SRM_EINVAL;
+    auto r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_ms < 0)
+        S->cv.wait(lk, pred);
+    else
+        S->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred);
+    return r->done ? 1 : 0;
+}
+
+/* copies the request's records (if out is large enough) and stats; returns the
+ * records' size in bytes.  release != 0 forgets the request afterwards. */
+long long srm_server_result(srm_server *S, long long id, char *out, size_t outcap,
+                            srm_request_stats *st, int release)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find((uint64_t)id);
+    if (it == S->all.end() || !it->second->done) return SRM_EINVAL;
+    auto r = it->second;
+    if (st) *st = r->st;
+    long long need = (long long)r->records.size();
+    if (out && (size_t)need <= outcap) std::memcpy(out, r->records.data(), (size_t)need);
+    if (release && out && (size_t)need <= outcap) S->all.erase(it);
+    return need;
+}
+
+void srm_server_destroy(srm_server *S)
+{
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    for (auto &t : S->threads) t.join();
+    for (size_t i = S->ctx.size(); i-- > 0;) {
+        srm_job_destroy(S->job[i]);
+        srm_ctx_destroy(S->ctx[i]);
+    }
+    srm_index_destroy(S->index);
+    delete S;
+}
+
+}  // extern "C"

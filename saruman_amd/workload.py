@@ -28,35 +28,8 @@ from .batch import pack_codes
 _vp = C.c_void_p
 
 
-def _protos(lib):
-    if getattr(lib, "_wl_protos", False):
-        return lib
-    spec = [
-        ("srm_synth_genome", C.c_int, [C.c_uint64, C.c_size_t, C.c_double, _vp]),
-        ("srm_synth_repeats", C.c_longlong, [C.c_uint64, _vp, C.c_size_t, C.c_int, C.c_int,
-                                             C.c_int, C.c_int, C.c_double, C.c_double]),
-        ("srm_synth_reads", C.c_int, [C.c_uint64, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_int,
-                                      C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
-                                      C.c_double, _vp, _vp, _vp, _vp, _vp]),
-        ("srm_filter_params", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
-        ("srm_index_build", C.c_int, [_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.POINTER(_vp)]),
-        ("srm_index_destroy", None, [_vp]),
-        ("srm_filter_run", C.c_int, [_vp, _vp, C.c_size_t, _vp, _vp, _vp, _vp, _vp, C.c_size_t,
-                                     C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
-        ("srm_filter_sizes", C.c_int, [_vp, _vp, _vp, _vp]),
-        ("srm_filter_copy", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-        ("srm_filter_destroy", None, [_vp]),
-    ]
-    for name, res, args in spec:
-        f = getattr(lib, name)
-        f.restype = res
-        f.argtypes = args
-    lib._wl_protos = True
-    return lib
-
-
 def lib():
-    return _protos(_lib.load())
+    return _lib.load()
 
 
 def filter_params(L: int, k: int):
