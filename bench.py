@@ -88,6 +88,35 @@ def cpu_baseline(genome, reads, db, n_sample, mapped_per_cand):
                 candidates_per_s=cand_rate, seconds=res.seconds), res
 
 
+def serve_leg(args):
+    """Continuous batching over the resident 5 Mbp genome: Poisson arrivals, request
+    sizes log-uniform in 1,000-200,000 reads of 36-250 bp, k = ceil(5% L)."""
+    from saruman_amd import serve
+    from saruman_amd import workload as wl
+    cfg, genome, pool = wl.make_workload("serve", n_reads=400_000)
+    rng = np.random.default_rng(cfg["seed"])
+    sizes = serve.request_sizes(rng, args.serve_requests)
+    qs = serve.qs_for_lengths(36, 250)
+    with serve.Server(genome, qs, n_slots=args.serve_slots, chunk_value=cfg["chunk"]) as S:
+        serve.poisson_load(S, pool, 50.0, [2000, 2000], seed=1)     # warm the slots
+        res = serve.poisson_load(S, pool, args.serve_rate, sizes, seed=cfg["seed"])
+    st = res.stats
+    dev = sorted((s["t_admitted"], s["t_aligned"]) for s in st)
+    overlapped = sum(1 for j in range(1, len(dev)) if dev[j][0] < dev[j - 1][1])
+    out = {"workload": "5 Mbp resident genome; Poisson arrivals; request sizes log-uniform in "
+                       "1,000-200,000 reads of 36-250 bp; k=ceil(5% L)",
+           "requests": len(st), "rate_hz": args.serve_rate, "slots": args.serve_slots,
+           "p50_ms": res.p50 * 1e3, "p99_ms": res.p99 * 1e3,
+           "reads_per_s": res.reads / res.wall, "mapped_reads_per_s": res.mapped / res.wall,
+           "reads": res.reads, "wall_s": res.wall,
+           "device_intervals_overlapping": overlapped,
+           "mean_filter_ms": float(np.mean([(s["t_filtered"] - s["t_submit"]) for s in st])) * 1e3,
+           "mean_device_ms": float(np.mean([s["device_ms"] for s in st]))}
+    log(f"[bench] serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
+        f"{out['reads_per_s']:.0f} reads/s over {len(st)} requests")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +129,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore saruman_amd/tuning/*.json")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -128,6 +162,9 @@ def main():
         f"{n_batches} reference batches, {len(perfect_reads)} reads with perfect hits")
 
     ctx = sa.Context(0)
+    from saruman_amd import tune
+    tuning = None if args.no_tuning else tune.apply_cached(ctx)
+    log(f"[bench] launch tuning: {tuning if tuning else 'defaults'}")
     t0 = time.perf_counter()
     ctx.set_genome(wl.genome_words(genome))
     job = sa.Job(ctx)
@@ -196,6 +233,10 @@ def main():
         cpu, _ = cpu_baseline(genome, reads, db, args.cpu_sample, n_mapped / max(db.n, 1))
         log(f"[bench] cpu oracle: {cpu['candidates_per_s']:.0f} candidates/s on {cpu['cores']} threads")
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_leg(args)
+
     value = n_mapped / (ms_per_step / 1e3)
     out = {
         "metric": "reads mapped/sec (batched persistent banded kernel; GCUPS and the plain "
@@ -214,7 +255,9 @@ def main():
         "config": {"workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}",
                    "reads": int(reads.n), "candidates": int(db.n), "reads_mapped": int(n_mapped),
                    "hits": int(hit.sum()), "match": 0, "mismatch": 1, "parallelism": "1 GPU",
-                   "graph": not args.no_graph},
+                   "graph": not args.no_graph,
+                   "tuning": ({k: tuning[k] for k in ("blocks_per_cu", "merge_below")}
+                              if tuning else "defaults")},
         "gcups": ref_cells / (ms_per_step / 1e3) / 1e9,
         "band_gcups": bcells / (band_ms / 1e3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
@@ -230,6 +273,7 @@ def main():
         "host": {"generate_s": t_gen, "index_s": t_index, "filter_s": t_filter,
                  "upload_s": t_upload, "download_s": t_d2h},
         "e2e_reads_per_s_pcie": n_mapped / (t_upload + ms_per_step / 1e3 + t_d2h),
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -205,6 +205,23 @@ int srm_ctx_set_genome(srm_ctx *ctx, const uint32_t *words, size_t n_words);
 int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src);
 
 /*
+ * Launch shape of the banded kernel, chosen by the autotuner
+ * (saruman_amd/tune.py) and applied to jobs uploaded afterwards.
+ * blocks_per_cu[nw]: persistent 256-thread blocks per CU for band class nw
+ * (8*nw band cells per lane), 0 = as many as stay resident; merge_below: band
+ * classes narrower than this run on the kernel of this class (fewer launches).
+ */
+typedef struct srm_tuning {
+    int blocks_per_cu[9];
+    int merge_below;
+} srm_tuning;
+int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *tuning);   /* NULL = defaults */
+/* after an eager srm_job_run: grid, tiles and event time (ms) of band class nw */
+int srm_job_band_info(srm_job *job, int nw, int *blocks, int *ntiles, float *ms);
+/* resident 256-thread blocks per CU of the banded kernel for class nw */
+int srm_band_occupancy(int nw, int unit);
+
+/*
  * A job: one batch of candidates, uploaded once, runnable many times.
  * win_words == NULL -> windows come from the context's resident genome.
  * read_bytes / win_bytes serve the SRM_CAND_BYTES candidates (may be NULL).
@@ -241,6 +258,10 @@ int srm_format_record(const char *name, unsigned int pos, int flag, int k, int L
 /* slot rows (as the reference's kernel leaves them) from a transcript; cap = w + k */
 int srm_slot_rows(const srm_result *res, const uint32_t *ops, const char *read, const char *win,
                   char *R, char *G, int cap);
+
+/* one streaming copy of `bytes` bytes on the device (reads `bytes`, writes
+ * `bytes`); used to calibrate hardware traffic counters.  ms = kernel time. */
+int srm_calibrate_copy(int device, size_t bytes, float *ms);
 
 /* pack ASCII ACGT into 2-bit codes; returns the number of non-ACGT bytes seen
  * (those are packed as A) */
@@ -305,6 +326,16 @@ int srm_synth_reads(uint64_t seed, const char *genome, const uint64_t *contig_lo
                     double p_random, char *out, uint64_t *off, uint16_t *len, int64_t *true_pos,
                     uint8_t *strand);
 
+/* FASTA / FASTQ input (src/util.c:293-454, src/saruman.c:139-271; extended to
+ * multi-line FASTA, every record a contig, reads of any length) */
+typedef struct srm_fastx srm_fastx;
+int srm_fastx_load(const char *path, int kind, int fixed_len, double n_rate, int n_mode,
+                   srm_fastx **out);
+int srm_fastx_sizes(const srm_fastx *x, uint64_t *n_records, uint64_t *n_bases,
+                    uint64_t *skipped_short, uint64_t *skipped_n);
+int srm_fastx_copy(const srm_fastx *x, char *seq, uint64_t *off, uint32_t *len, char *names);
+void srm_fastx_destroy(srm_fastx *x);
+
 /* ===================================================================== */
 /* Part 4. Serving: continuous batching over a resident genome            */
 /* ===================================================================== */
@@ -325,6 +356,7 @@ typedef struct srm_request_stats {
     double t_submit, t_filtered, t_admitted, t_aligned, t_done;   /* s since server start */
     double device_ms;
     long long n_reads, n_candidates, n_hits, n_perfect, n_mapped;
+    long long dev_bytes;   /* device footprint charged against the HBM budget */
     int slot, status;
 } srm_request_stats;
 
@@ -336,6 +368,10 @@ long long srm_server_submit(srm_server *s, const char *reads, const uint64_t *re
 int srm_server_wait(srm_server *s, long long id, int timeout_ms);
 long long srm_server_result(srm_server *s, long long id, char *out, size_t outcap,
                             srm_request_stats *stats, int release);
+/* on != 0 pauses admission (filtered requests wait; running ones finish); 0 resumes */
+void srm_server_hold(srm_server *s, int on);
+/* queue depths: not yet filtered, waiting for admission, on a slot */
+void srm_server_queue(srm_server *s, uint64_t *incoming, uint64_t *ready, uint64_t *running);
 void srm_server_destroy(srm_server *s);
 
 #ifdef __cplusplus

@@ -54,6 +54,11 @@ _PROTOS = [
     ("srm_slot_rows", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     ("srm_pack_codes", C.c_longlong, [_vp, C.c_size_t, _vp, C.c_size_t]),
     ("srm_ctx_borrow_genome", C.c_int, [_vp, _vp]),
+    ("srm_ctx_set_tuning", C.c_int, [_vp, _vp]),
+    ("srm_job_band_info", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_float)]),
+    ("srm_band_occupancy", C.c_int, [C.c_int, C.c_int]),
+    ("srm_calibrate_copy", C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_float)]),
     ("srm_format_records", C.c_longlong, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                           C.c_char_p, _vp, C.c_size_t, C.POINTER(C.c_longlong),
                                           C.c_int]),
@@ -72,11 +77,18 @@ _PROTOS = [
     ("srm_synth_reads", C.c_int, [C.c_uint64, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_int,
                                   C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                   C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    ("srm_fastx_load", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.POINTER(_vp)]),
+    ("srm_fastx_sizes", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("srm_fastx_copy", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("srm_fastx_destroy", None, [_vp]),
     # serving
     ("srm_server_create", C.c_int, [_vp, _vp, C.c_size_t, _vp, C.c_int, C.POINTER(_vp)]),
     ("srm_server_submit", C.c_longlong, [_vp, _vp, _vp, _vp, _vp, C.c_size_t]),
     ("srm_server_wait", C.c_int, [_vp, C.c_longlong, C.c_int]),
     ("srm_server_result", C.c_longlong, [_vp, C.c_longlong, _vp, C.c_size_t, _vp, C.c_int]),
+    ("srm_server_hold", None, [_vp, C.c_int]),
+    ("srm_server_queue", None, [_vp, _vp, _vp, _vp]),
     ("srm_server_destroy", None, [_vp]),
     # drop-in entry points (src/cuda-functions.h:55-78)
     ("cuda_needleman_wunsch", None, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp,

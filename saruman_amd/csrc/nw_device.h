@@ -35,12 +35,12 @@ struct BandArgs {
     int ma, mm;
     const srm_cand *cand;
     const uint32_t *perm;           // candidate indices, tile-major
-    const uint4 *tiles;             // x perm start, y count (<= 64), z max L, w unused
+    const uint4 *tiles;             // x perm start, y count (<= 64), z max L, w max k
     uint32_t ntiles;
     uint32_t *queue;                // tile counter (zeroed before the launch)
     const uint32_t *read_words;
     const uint32_t *win_words;
-    uint32_t *scratch;              // per wave: (maxL + 1) * NW * 64 words
+    uint32_t *scratch;              // per wave: 8*NW * ceil((maxL + 1) / 8) * 64 words
     size_t scratch_words_per_wave;
     srm_result *res;
     uint32_t *ops;
@@ -50,5 +50,6 @@ hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st);
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
 }  // namespace srm

@@ -168,15 +168,44 @@ __global__ __launch_bounds__(256) void plain_kernel(PlainArgs a)
 // ---------------------------------------------------------------------------
 // banded persistent kernel
 // ---------------------------------------------------------------------------
-// nibble (value) of band cell i in column col, from the wave's scratch
-__device__ __forceinline__ int nib(const uint32_t *__restrict__ scr, int NW, int col, int i, int lane)
+// The wave's scratch holds the band values as nibbles in blocks of 8 columns:
+// word (q, i) of lane l sits at ((q * BMAX + i) * 64 + l) and holds band cell
+// (diagonal) i of columns 8q .. 8q+7 (column 8q + j in bits 4j..4j+3).  The
+// forward pass fills the BMAX words of a block every 8 columns with coalesced
+// 256-byte wave stores at immediate offsets; a traceback step along a diagonal
+// stays inside the words it already has.
+template <int BMAX>
+__device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, int i, int q,
+                                              int lane)
 {
-    uint32_t word = scr[((size_t)col * NW + (i >> 3)) * 64 + lane];
-    return (int)((word >> ((i & 7) * 4)) & 15u);
+    return scr[((size_t)q * BMAX + i) * 64 + lane];
 }
 
+// 16 codes of a sequence starting at a multiple of 16 at or below idx
+struct CodeCache {
+    int64_t base = INT64_MIN;
+    uint32_t word = 0;
+    __device__ __forceinline__ int at(const uint32_t *__restrict__ words, int64_t org, int idx)
+    {
+        const int64_t b = (int64_t)(idx & ~15);
+        if (b != base) {
+            base = b;
+            word = fetch16(words, org + b);
+        }
+        return (int)((word >> ((idx & 15) * 2)) & 3u);
+    }
+};
+
+// occupancy each band class is compiled for (waves per SIMD): the register
+// budget is 512 / waves VGPRs
+template <int NW>
+struct BandWaves {
+    static constexpr int value = NW <= 1 ? 8 : NW <= 2 ? 5 : NW <= 3 ? 4 : NW <= 5 ? 3 : 2;
+};
+
 template <int NW, bool UNIT>
-__global__ __launch_bounds__(256) void band_kernel(BandArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BandWaves<NW>::value)))
+void band_kernel(BandArgs a)
 {
     constexpr int BMAX = NW * 8;            // band cells per column held in VGPRs
     constexpr int WW = (BMAX + 15) / 16;    // code words of the sliding window
@@ -190,7 +219,7 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
         if (lane == 0) tile = atomicAdd(a.queue, 1u);
         tile = __builtin_amdgcn_readfirstlane(tile);
         if (tile >= a.ntiles) break;
-        const uint4 td = a.tiles[tile];     // x: perm start, y: count, z: max L
+        const uint4 td = a.tiles[tile];     // x: perm start, y: count, z: max L, w: max k
         const bool active = (uint32_t)lane < td.y;
         const uint32_t t = active ? a.perm[td.x + lane] : 0u;
         srm_cand cd = a.cand[t];
@@ -200,13 +229,22 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
         const int dlo = -k;
         const int K = k + 1;
         const int Lmax = (int)td.z;
+        // unit costs: a band value grows by at most 1 per column, so saturating
+        // every (15 - max k) columns keeps every stored value <= 15
+        const int satp = UNIT ? 15 - (int)td.w : 1;
 
         // ---- column 0: M[h][0] = max(0, h - 2s) (unit steps, Q7); rows < 0 -> K
+        // Unit costs run in the potential W = M - 2c - i (column c, band cell i): the
+        // left and top moves then cost 0 and the diagonal costs bit - 2, so a cell is
+        // min3(W + bit - 2, W_left, W_up).  Other costs run on M itself.
         int v[BMAX];
+        uint32_t acc[BMAX];
 #pragma unroll
         for (int i = 0; i < BMAX; ++i) {
             const int h = dlo + i;
-            v[i] = h < 0 ? K : imin(imax(h - 2 * s, 0), K);
+            const int m0 = h < 0 ? K : imin(imax(h - 2 * s, 0), K);
+            v[i] = UNIT ? m0 - i : m0;
+            acc[i] = (uint32_t)v[i];
         }
         // window codes for column 1: W[i] = g[dlo + i]; a 16*WW-code window plus a stream
         uint32_t gw[WW];
@@ -216,17 +254,7 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
         uint32_t gnext = 0;   // stream of window codes entering the top of the window
         uint32_t rbuf = 0;    // stream of read codes
         const int64_t rbase = (int64_t)cd.read_off;
-
-        auto store_col = [&](int col) {
-#pragma unroll
-            for (int j = 0; j < NW; ++j) {
-                uint32_t acc = (uint32_t)v[8 * j];
-#pragma unroll
-                for (int q = 1; q < 8; ++q) acc |= (uint32_t)v[8 * j + q] << (4 * q);
-                scr[((size_t)col * NW + j) * 64 + lane] = acc;
-            }
-        };
-        if (active) store_col(0);
+        int since_sat = 0;
 
         // one DP column; last selects the free trailing verticals of column L
         auto column = [&](int c, bool last) {
@@ -250,32 +278,78 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
             }
             int ifree = BMAX;            // first band cell whose top move is free
             if (last) ifree = w - 2 * e + 1 - c - dlo;
-            int up = 1 << 20;            // new value of cell i-1 (out of band above)
+            const bool sat = !UNIT || last || ++since_sat >= satp;
+            if (sat) since_sat = 0;
+            const uint32_t sh = (uint32_t)(c & 7) * 4u;
+            if (UNIT) {
+                uint32_t misf[WW];
 #pragma unroll
-            for (int i = 0; i < BMAX; ++i) {
-                const int bit = (int)((mis[i >> 4] >> (2 * (i & 15))) & 1u);
-                const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 20);
-                int nv;
-                if (UNIT && !last) {
-                    // min(v + bit, lf + 1, up + 1) = min3(v + bit - 1, lf, up) + 1
-                    nv = imin(imin(v[i] + bit - 1, lf), up) + 1;
-                } else {
+                for (int j = 0; j < WW; ++j) misf[j] = mis[j] | 0xAAAAAAAAu;
+                int up = 1 << 28;
+                const int satv = K - 2 * c;   // W bound of cell i: K - 2c - i
+#pragma unroll
+                for (int i = 0; i < BMAX; ++i) {
+                    // field i of misf is 0b10 (match) or 0b11 (mismatch): read signed it
+                    // is the diagonal cost in W, -2 or -1, in one v_bfe_i32
+                    const int dcost = __builtin_amdgcn_sbfe((int)misf[i >> 4], 2 * (i & 15), 2);
+                    const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 28);
+                    // the top move into a row of the free tail of column L costs 0: -1 in W
+                    const int upc = (last && i >= ifree) ? up - 1 : up;
+                    int nv = imin(imin(v[i] + dcost, lf), upc);
+                    if (last) nv = imin(nv, satv - i);
+                    v[i] = nv;
+                    up = nv;
+                }
+                if (sat && !last) {
+#pragma unroll
+                    for (int i = 0; i < BMAX; ++i) v[i] = imin(v[i], satv - i);
+                }
+            } else {
+                int up = 1 << 20;
+#pragma unroll
+                for (int i = 0; i < BMAX; ++i) {
+                    const int bit = (int)((mis[i >> 4] >> (2 * (i & 15))) & 1u);
+                    const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 20);
                     const int dg = v[i] + (bit ? mm : ma);
                     const int tcost = (last && i >= ifree) ? 0 : mm;
-                    nv = imin(imin(dg, lf + mm), up + tcost);
+                    int nv = imin(imin(dg, lf + mm), up + tcost);
+                    nv = imin(nv, K);
+                    v[i] = nv;
+                    up = nv;
                 }
-                nv = imin(nv, K);
-                v[i] = nv;
-                up = nv;
+                if (c <= k) {
+                    // row 0 keeps its unit-step init M[0][c] = c (Q7) whatever the mismatch cost
+                    const int i0 = k - c;
+#pragma unroll
+                    for (int i = 0; i < BMAX; ++i) v[i] = (i == i0) ? imin(c, K) : v[i];
+                }
             }
-            if (!UNIT && c <= k) {
-                // row 0 keeps its unit-step init M[0][c] = c (Q7) whatever the mismatch cost
-                const int i0 = k - c;
+            // nibble c & 7 of every diagonal's word; a block is stored when complete
+            if ((c & 7) == 0) {
+#pragma unroll
+                for (int i = 0; i < BMAX; ++i) acc[i] = (uint32_t)v[i];
+            } else {
 #pragma unroll
                 for (int i = 0; i < BMAX; ++i)
-                    if (i == i0) v[i] = imin(c, K);
+                    acc[i] = UNIT ? ((uint32_t)v[i] << sh) + acc[i] : ((uint32_t)v[i] << sh) | acc[i];
             }
-            store_col(c);
+            if ((c & 7) == 7 || last) {
+                uint32_t *__restrict__ blk = scr + (size_t)(c >> 3) * BMAX * 64 + lane;
+                if (UNIT) {
+                    // back from W: nibble j of word (q, i) gets 2(8q + j) + i, i.e. the
+                    // word gets (16q + i) * sum_j 16^j + 2 * sum_j j 16^j over the
+                    // nibbles written so far (exact modulo 2^32)
+                    const uint32_t msk = (c & 7) == 7 ? 0xFFFFFFFFu : ((4u << sh) << 2) - 1u;
+                    const uint32_t ones = 0x11111111u & msk;
+                    const uint32_t base = 0xECA86420u & msk;
+                    const uint32_t q16 = (uint32_t)(c >> 3) * 16u;
+#pragma unroll
+                    for (int i = 0; i < BMAX; ++i) blk[i * 64] = acc[i] + (q16 + (uint32_t)i) * ones + base;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < BMAX; ++i) blk[i * 64] = acc[i];
+                }
+            }
         };
 
         for (int c = 1; c < Lmax; ++c) {
@@ -288,7 +362,10 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
             srm_result r;
             const int iend = w - L - dlo;
             int score = K;
-            if (iend >= 0 && iend < BMAX) score = nib(scr, NW, L, iend, lane);
+            if (iend >= 0 && iend < BMAX) {
+                const uint32_t wd0 = band_word<BMAX>(scr, iend, L >> 3, lane);
+                score = imin((int)((wd0 >> ((L & 7) * 4)) & 15u), K);
+            }
             r.score = (int16_t)score;
             r.n_ops = 0;
             r.end_row = (uint16_t)w;
@@ -298,35 +375,65 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
                 uint32_t *__restrict__ ops = a.ops + cd.ops_off;
                 const int cap = w + k;
                 int row = w, col = L, n = 0;
-                uint32_t acc = 0;
+                uint32_t opw = 0;
+                // cached words of diagonals ci-1, ci, ci+1 in column block cq (Wm, W0,
+                // Wp) and of ci, ci+1 in block cq-1 (P0, Pp)
+                int ci = -1, cq = -1;
+                uint32_t Wm = 0, W0 = 0, Wp = 0, P0 = 0, Pp = 0;
+                CodeCache rcc, gcc;
                 for (;;) {
                     if (stop_row0 && row == 0) break;
                     const int i = row - col - dlo;
-                    const int vv = nib(scr, NW, col, i, lane);
+                    const int q = col >> 3;
+                    if (i != ci || q != cq) {
+                        W0 = band_word<BMAX>(scr, i, q, lane);
+                        Wm = i >= 1 ? band_word<BMAX>(scr, i - 1, q, lane) : 0xFFFFFFFFu;
+                        Wp = i + 1 < BMAX ? band_word<BMAX>(scr, i + 1, q, lane) : 0xFFFFFFFFu;
+                        if (q >= 1) {
+                            P0 = band_word<BMAX>(scr, i, q - 1, lane);
+                            Pp = i + 1 < BMAX ? band_word<BMAX>(scr, i + 1, q - 1, lane) : 0xFFFFFFFFu;
+                        }
+                        ci = i;
+                        cq = q;
+                    }
+                    const uint32_t sh = (uint32_t)(col & 7) * 4u;
+                    // a nibble of 15 outside the band reads as K (never equal to v <= k)
+                    const int vv = (int)((W0 >> sh) & 15u);
                     int Dg, Lf, Tp;
                     if (col != 0) {
-                        Dg = row >= 1 ? nib(scr, NW, col - 1, i, lane) : SRM_FOREIGN;
-                        Lf = (i + 1 < BMAX) ? nib(scr, NW, col - 1, i + 1, lane) : K;
+                        uint32_t dgw, lfw;
+                        uint32_t psh;
+                        if (col & 7) {
+                            dgw = W0;
+                            lfw = Wp;
+                            psh = sh - 4u;
+                        } else {
+                            dgw = P0;
+                            lfw = Pp;
+                            psh = 28u;
+                        }
+                        Dg = row >= 1 ? imin((int)((dgw >> psh) & 15u), K) : SRM_FOREIGN;
+                        Lf = imin((int)((lfw >> psh) & 15u), K);
                     } else {
                         Dg = SRM_COL0;
                         Lf = SRM_COL0;
                     }
-                    Tp = row >= 1 ? (i >= 1 ? nib(scr, NW, col, i - 1, lane) : K) : SRM_FOREIGN;
+                    Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
                     const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
-                    const bool eq = code_at(a.win_words, (int64_t)cd.win_off + ht) ==
-                                    code_at(a.read_words, rbase + wd);
+                    const bool eq = gcc.at(a.win_words, (int64_t)cd.win_off, ht) ==
+                                    rcc.at(a.read_words, rbase, wd);
                     const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
                     if (op == 3 || n >= cap) break;
-                    acc |= (uint32_t)op << ((n & 15) * 2);
+                    opw |= (uint32_t)op << ((n & 15) * 2);
                     ++n;
                     if ((n & 15) == 0) {
-                        ops[(n >> 4) - 1] = acc;
-                        acc = 0;
+                        ops[(n >> 4) - 1] = opw;
+                        opw = 0;
                     }
                     row -= (op != 2);
                     col -= (op != 1);
                 }
-                if (n & 15) ops[n >> 4] = acc;
+                if (n & 15) ops[n >> 4] = opw;
                 r.n_ops = (uint16_t)n;
                 r.end_row = (uint16_t)row;
                 r.end_col = (uint16_t)col;
@@ -334,6 +441,23 @@ __global__ __launch_bounds__(256) void band_kernel(BandArgs a)
             a.res[t] = r;
         }
     }
+}
+
+// streaming copy of n 16-byte words (calibrates the HBM traffic counters)
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst,
+                       n);
+    return hipGetLastError();
 }
 
 // clears the queue counters of a launch sequence (a kernel, not hipMemsetAsync,

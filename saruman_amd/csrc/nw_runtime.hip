@@ -84,6 +84,7 @@ struct srm_ctx {
     }
     std::mutex mu;
     srm_job *dropin_job = nullptr;
+    srm_tuning tuning{};      // zero = defaults (occupancy-sized grids, no merging)
 
     int ensure(DevBuf &b, size_t bytes)
     {
@@ -154,8 +155,10 @@ struct srm_job {
     size_t n_band = 0, n_rest = 0, n_bytes = 0;
     size_t mat_budget_elems = 0;
     // timing
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[3] = {};
+    hipEvent_t cev[9] = {};          // around each band-class launch
     float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f;
+    float class_ms[9] = {};          // by NW (1..8), last eager run
     // graph
     bool graph_enabled = false;
     hipGraph_t graph = nullptr;
@@ -173,6 +176,12 @@ inline int band_cells(const srm_cand &c)
     int s = (c.flag <= 7 && (c.flag & 2)) ? k : 0;
     int hi = std::max(2 * s + k, (int)c.w - (int)c.L);
     return hi + k + 1;
+}
+
+// words of one wave's band scratch: 8*NW diagonals x 8-column blocks x 64 lanes
+inline size_t band_scratch_words(const BandClass &bc)
+{
+    return (size_t)bc.nw * 8 * (size_t)((bc.maxL + 8) / 8) * 64;
 }
 
 void destroy_graph(srm_job *j)
@@ -247,10 +256,12 @@ int run_launches(srm_job *j, int kernel, bool timed)
                               ? ctx->genome_ptr()
                               : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
             a.scratch = (uint32_t *)j->d_scratch.p;
-            a.scratch_words_per_wave = (size_t)(bc.maxL + 1) * bc.nw * 64;
+            a.scratch_words_per_wave = band_scratch_words(bc);
             a.res = (srm_result *)j->d_res.p;
             a.ops = (uint32_t *)j->d_ops.p;
+            if (timed && c == 0) HIP_TRY(hipEventRecord(j->cev[0], st));
             HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, st));
+            if (timed) HIP_TRY(hipEventRecord(j->cev[c + 1], st));
         }
     }
     if (timed) HIP_TRY(hipEventRecord(j->ev[1], st));
@@ -363,6 +374,35 @@ int srm_ctx_set_genome(srm_ctx *c, const uint32_t *words, size_t n_words)
     return SRM_OK;
 }
 
+int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *t)
+{
+    if (!t) {
+        ctx->tuning = srm_tuning{};
+        return SRM_OK;
+    }
+    for (int i = 0; i < 9; ++i)
+        if (t->blocks_per_cu[i] < 0 || t->blocks_per_cu[i] > 64) return fail(SRM_EINVAL, "blocks_per_cu");
+    if (t->merge_below < 0 || t->merge_below > 8) return fail(SRM_EINVAL, "merge_below");
+    ctx->tuning = *t;
+    return SRM_OK;
+}
+
+int srm_job_band_info(srm_job *j, int nw, int *blocks, int *ntiles, float *ms)
+{
+    if (nw < 1 || nw > 8) return fail(SRM_EINVAL, "nw");
+    *blocks = 0;
+    *ntiles = 0;
+    *ms = j->class_ms[nw];
+    for (const auto &bc : j->classes)
+        if (bc.nw == nw) {
+            *blocks = bc.blocks;
+            *ntiles = (int)bc.ntiles;
+        }
+    return SRM_OK;
+}
+
+int srm_band_occupancy(int nw, int unit) { return srm::band_occupancy(nw, unit != 0); }
+
 int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src)
 {
     if (!src->genome_ptr()) return fail(SRM_ESTATE, "source context has no genome");
@@ -378,10 +418,13 @@ int srm_job_create(srm_ctx *ctx, srm_job **out)
     HIP_TRY(hipSetDevice(ctx->device));
     srm_job *j = new srm_job();
     j->ctx = ctx;
-    for (auto &e : j->ev) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete j;
-            return fail(SRM_EDEVICE, "hipEventCreate failed");
+    for (auto *arr : {&j->ev[0], &j->cev[0]}) {
+        const int cnt = arr == &j->ev[0] ? 3 : 9;
+        for (int i = 0; i < cnt; ++i) {
+            if (hipEventCreate(&arr[i]) != hipSuccess) {
+                delete j;
+                return fail(SRM_EDEVICE, "hipEventCreate failed");
+            }
         }
     }
     *out = j;
@@ -401,6 +444,8 @@ void srm_job_destroy(srm_job *j)
                       &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops};
     for (DevBuf *b : bufs) c->release(*b);
     for (auto &e : j->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : j->cev)
         if (e) (void)hipEventDestroy(e);
     delete j;
 }
@@ -471,7 +516,8 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
 
     // ---- banded plan: sort by (class, L desc), 64-candidate tiles
     std::vector<int> cls(n, 0);
-    for (uint32_t t : band_idx) cls[t] = (band_cells(cand[t]) + 7) / 8;
+    const int merge_below = std::max(1, std::min(8, ctx->tuning.merge_below));
+    for (uint32_t t : band_idx) cls[t] = std::max((band_cells(cand[t]) + 7) / 8, merge_below);
     std::stable_sort(band_idx.begin(), band_idx.end(), [&](uint32_t a, uint32_t b) {
         if (cls[a] != cls[b]) return cls[a] < cls[b];
         return cand[a].L > cand[b].L;
@@ -490,19 +536,22 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             while (e < band_idx.size() && cls[band_idx[e]] == bc.nw) ++e;
             for (size_t s = i; s < e; s += 64) {
                 uint32_t cnt = (uint32_t)std::min<size_t>(64, e - s);
+                uint32_t kmax = 0;
+                for (size_t u = s; u < s + cnt; ++u) kmax = std::max<uint32_t>(kmax, cand[band_idx[u]].k);
                 tiles.push_back((uint32_t)s);
                 tiles.push_back(cnt);
-                tiles.push_back(cand[band_idx[s]].L);
-                tiles.push_back(0);
+                tiles.push_back(cand[band_idx[s]].L);   // sorted by L: the tile's max
+                tiles.push_back(kmax);
             }
             bc.ntiles = (uint32_t)(tiles.size() / 4) - bc.tile_begin;
             // persistent grid: as many 4-wave blocks as stay resident on every CU
             // (occupancy of this kernel instance), never more waves than tiles
-            const int per_cu = srm::band_occupancy(bc.nw, match == 0 && mismatch == 1);
+            int per_cu = srm::band_occupancy(bc.nw, match == 0 && mismatch == 1);
+            const int tuned = ctx->tuning.blocks_per_cu[bc.nw];
+            if (tuned > 0) per_cu = std::min(per_cu, tuned);
             int max_blocks = std::max(1, ctx->num_cus * per_cu);
             bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
-            scratch_words = std::max(scratch_words, (size_t)bc.blocks * 4 * (size_t)(bc.maxL + 1) *
-                                                        bc.nw * 64);
+            scratch_words = std::max(scratch_words, (size_t)bc.blocks * 4 * band_scratch_words(bc));
             j->classes.push_back(bc);
             i = e;
         }
@@ -586,6 +635,9 @@ int srm_job_run(srm_job *j, int kernel)
     HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[2]));
     HIP_TRY(hipEventElapsedTime(&j->band_ms, j->ev[0], j->ev[1]));
     HIP_TRY(hipEventElapsedTime(&j->plain_ms, j->ev[1], j->ev[2]));
+    for (auto &x : j->class_ms) x = 0.f;
+    for (size_t c = 0; c < j->classes.size(); ++c)
+        HIP_TRY(hipEventElapsedTime(&j->class_ms[j->classes[c].nw], j->cev[c], j->cev[c + 1]));
     return SRM_OK;
 }
 
@@ -620,6 +672,34 @@ int srm_job_path_count(srm_job *j, int path)
     if (path == 0) return (int)j->n_band;
     if (path == 1) return (int)j->n_rest;
     return (int)j->n_bytes;
+}
+
+int srm_calibrate_copy(int device, size_t bytes, float *ms)
+{
+    HIP_TRY(hipSetDevice(device));
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(SRM_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMemset(a, 1, bytes));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    HIP_TRY(srm::launch_copy(a, b, bytes, nullptr));
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return SRM_OK;
 }
 
 long long srm_pack_codes(const char *seq, size_t n, uint32_t *words, size_t code_off)

@@ -84,6 +84,7 @@ struct srm_server {
     size_t budget_used = 0;
     uint64_t next_id = 1;
     bool stop = false;
+    bool hold = false;                // admission paused (srm_server_hold)
     std::vector<std::thread> threads;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -158,6 +159,7 @@ void srm_server::prepare(Request &r)
                   r.ops_total * 4 + waves * (maxL + 1) * 8 * 64 * 4 + (1u << 20);
     r.st.n_reads = (long long)n;
     r.st.n_candidates = (long long)nc;
+    r.st.dev_bytes = (long long)r.dev_bytes;
 }
 
 void srm_server::run_on_slot(int s, Request &r)
@@ -259,7 +261,7 @@ void srm_server::admit_loop()
         int slot = -1;
         cv.wait(lk, [&] {
             if (stop) return true;
-            if (ready.empty()) return false;
+            if (hold || ready.empty()) return false;
             if (budget_used + ready.front()->dev_bytes > cfg.hbm_budget) return false;
             for (size_t s = 0; s < slot_busy.size(); ++s)
                 if (!slot_busy[s]) {
@@ -416,6 +418,31 @@ long long srm_server_result(srm_server *S, long long id, char *out, size_t outca
     if (out && (size_t)need <= outcap) std::memcpy(out, r->records.data(), (size_t)need);
     if (release && out && (size_t)need <= outcap) S->all.erase(it);
     return need;
+}
+
+/* on != 0 pauses admission: filtered requests wait in the ready queue and
+ * requests already on a slot finish; on == 0 resumes it. */
+void srm_server_hold(srm_server *S, int on)
+{
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->hold = on != 0;
+    }
+    S->cv.notify_all();
+}
+
+/* queue depths: submitted and not yet filtered, filtered and waiting for
+ * admission, on a slot */
+void srm_server_queue(srm_server *S, uint64_t *incoming, uint64_t *ready, uint64_t *running)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (incoming) *incoming = S->incoming.size();
+    if (ready) *ready = S->ready.size();
+    if (running) {
+        uint64_t n = 0;
+        for (auto &q : S->slot_req) n += q != nullptr;
+        *running = n;
+    }
 }
 
 void srm_server_destroy(srm_server *S)

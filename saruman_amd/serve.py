@@ -27,8 +27,8 @@ class RequestStats(C.Structure):
                 ("t_admitted", C.c_double), ("t_aligned", C.c_double), ("t_done", C.c_double),
                 ("device_ms", C.c_double), ("n_reads", C.c_longlong),
                 ("n_candidates", C.c_longlong), ("n_hits", C.c_longlong),
-                ("n_perfect", C.c_longlong), ("n_mapped", C.c_longlong), ("slot", C.c_int),
-                ("status", C.c_int)]
+                ("n_perfect", C.c_longlong), ("n_mapped", C.c_longlong),
+                ("dev_bytes", C.c_longlong), ("slot", C.c_int), ("status", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -77,6 +77,16 @@ class Server:
         if st.status != 0:
             raise RuntimeError(f"request {rid} failed with status {st.status}")
         return buf.raw[:need].decode(), st.as_dict()
+
+    def hold(self, on: bool = True) -> None:
+        """Pauses (True) or resumes (False) admission; running requests finish."""
+        self.lib.srm_server_hold(self.h, int(on))
+
+    def queue(self):
+        """(not yet filtered, waiting for admission, on a slot)"""
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.lib.srm_server_queue(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
 
     def close(self):
         if self.h:

@@ -1,0 +1,147 @@
+"""Autotuner for the banded kernel's launch shape on this GPU.
+
+For every band class (8*nw band cells per lane, nw = 1..8) it measures the
+persistent kernel with 1..max resident blocks per CU, each class timed with HIP
+events of its own launch, then tries running the narrow classes on a wider
+kernel (fewer launches, fewer tails).  The winner is cached in
+saruman_amd/tuning/<arch>.json and applied with srm_ctx_set_tuning().
+
+    python -m saruman_amd.tune [--reads N]     (on the GPU)
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+
+TUNING_DIR = Path(__file__).resolve().parent / "tuning"
+ARCH = "gfx950"
+
+
+class Tuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int * 9), ("merge_below", C.c_int)]
+
+
+def table_path(arch: str = ARCH) -> Path:
+    return TUNING_DIR / f"{arch}.json"
+
+
+def load_table(arch: str = ARCH):
+    p = table_path(arch)
+    if not p.exists():
+        return None
+    return json.loads(p.read_text())
+
+
+def apply(ctx, table) -> None:
+    """Applies a tuning table (dict) to a saruman_amd.Context; None = defaults."""
+    lib = _lib.load()
+    if table is None:
+        _lib.check(lib.srm_ctx_set_tuning(ctx.h, None), "srm_ctx_set_tuning")
+        return
+    t = Tuning()
+    for i, b in enumerate(table["blocks_per_cu"][:9]):
+        t.blocks_per_cu[i] = int(b)
+    t.merge_below = int(table.get("merge_below", 0))
+    _lib.check(lib.srm_ctx_set_tuning(ctx.h, C.byref(t)), "srm_ctx_set_tuning")
+
+
+def apply_cached(ctx, arch: str = ARCH):
+    table = load_table(arch)
+    if table is not None:
+        apply(ctx, table)
+    return table
+
+
+def _class_times(job, reps):
+    lib = _lib.load()
+    per = {nw: [] for nw in range(1, 9)}
+    totals = []
+    for _ in range(reps):
+        job.run("banded")
+        b_ms, _ = job.kernel_ms()
+        totals.append(b_ms)
+        for nw in range(1, 9):
+            blocks, ntiles, ms = C.c_int(), C.c_int(), C.c_float()
+            lib.srm_job_band_info(job.h, nw, C.byref(blocks), C.byref(ntiles), C.byref(ms))
+            if ntiles.value:
+                per[nw].append(ms.value)
+    return float(np.median(totals)), {nw: float(np.median(v)) for nw, v in per.items() if v}
+
+
+def tune(ctx, upload, reps: int = 3, log=print):
+    """upload(job) uploads the representative candidate batch into a Job."""
+    from .batch import Job
+    lib = _lib.load()
+    job = Job(ctx)
+    try:
+        apply(ctx, None)
+        upload(job)
+        base_total, base = _class_times(job, reps)
+        log(f"[tune] default launch: band kernels {base_total:.3f} ms "
+            + " ".join(f"nw{nw}={ms:.3f}" for nw, ms in base.items()))
+        occ = {nw: lib.srm_band_occupancy(nw, 1) for nw in range(1, 9)}
+        best = {nw: (ms, 0) for nw, ms in base.items()}
+        for b in range(1, max(occ.values()) + 1):
+            t = {"blocks_per_cu": [0] + [min(b, occ[nw]) for nw in range(1, 9)], "merge_below": 0}
+            apply(ctx, t)
+            upload(job)
+            tot, per = _class_times(job, reps)
+            log(f"[tune] {b} block(s)/CU: {tot:.3f} ms " +
+                " ".join(f"nw{nw}={ms:.3f}" for nw, ms in per.items()))
+            for nw, ms in per.items():
+                if b <= occ[nw] and ms < best.get(nw, (1e30, 0))[0]:
+                    best[nw] = (ms, b)
+        blocks = [0] + [best.get(nw, (0, 0))[1] for nw in range(1, 9)]
+        results = {}
+        for mb in (0, 2, 3):
+            t = {"blocks_per_cu": blocks, "merge_below": mb}
+            apply(ctx, t)
+            upload(job)
+            tot, _ = _class_times(job, reps)
+            results[mb] = tot
+            log(f"[tune] merge_below={mb}: {tot:.3f} ms")
+        mb = min(results, key=results.get)
+        table = {"arch": ARCH, "blocks_per_cu": blocks, "merge_below": mb,
+                 "band_ms_default": base_total, "band_ms_tuned": results[mb],
+                 "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S")}
+        apply(ctx, table)
+        return table
+    finally:
+        job.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reads", type=int, default=500_000)
+    ap.add_argument("--workload", default="mixed")
+    ap.add_argument("--out", default=str(table_path()))
+    args = ap.parse_args()
+    from . import workload as wl
+    from .batch import Context
+    cfg, genome, reads = wl.make_workload(args.workload, n_reads=args.reads)
+    index = wl.Index(genome, wl.needed_qs(reads))
+    strands = [wl.run_filter(index, genome, reads, dr, cfg["chunk"]) for dr in (0, 4)]
+    db = wl.device_batch(reads, strands)
+    ctx = Context(0)
+    ctx.set_genome(wl.genome_words(genome))
+
+    def upload(job):
+        job.upload(0, 1, db.cand, db.read_words, None, ops_total=db.ops_total)
+
+    table = tune(ctx, upload)
+    table["workload"] = f"{args.workload}, {args.reads} reads, {db.n} candidates"
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
+    print(json.dumps(table), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -142,3 +142,30 @@ def test_graph_replay(orc):
             sa.align_batch(random_batch(rng, 100, L=36, k=2), kernel="banded")   # eager work
     finally:
         job.close()
+
+
+def test_tuned_launch_shapes_keep_results(orc):
+    """Every launch shape the autotuner may pick gives the same results."""
+    banner("[gpu] tuning shapes")
+    from saruman_amd import tune
+    from saruman_amd.batch import Context, Job, _layout, pack_codes
+    rng = np.random.default_rng(15)
+    batch = random_batch(rng, 3000, L=(36, 250), k=13, per_cand_k=lambda L: math.ceil(0.05 * L))
+    lay = _layout(batch)
+    ctx = Context(0)
+    try:
+        outs = []
+        for table in (None, {"blocks_per_cu": [0] + [1] * 8, "merge_below": 3},
+                      {"blocks_per_cu": [0, 2, 1, 3, 1, 2, 1, 1, 1], "merge_below": 2}):
+            tune.apply(ctx, table)
+            job = Job(ctx)
+            job.upload(0, 1, lay["cand"], pack_codes(lay["rcat"]), pack_codes(lay["wcat"]),
+                       ops_total=lay["ops_total"])
+            job.run("banded")
+            outs.append(job.download())
+            job.close()
+        for res, ops in outs[1:]:
+            assert (res == outs[0][0]).all()
+            assert (ops == outs[0][1]).all()
+    finally:
+        ctx.close()
