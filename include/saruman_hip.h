@@ -316,6 +316,26 @@ int srm_filter_copy(const srm_filter *f, int32_t *cand_read, int64_t *cand_pos,
                     int32_t *perfect_read, int64_t *perfect_pos);
 void srm_filter_destroy(srm_filter *f);
 
+/*
+ * The same filter with the per-read work on the GPU (same replaced reference
+ * functions as srm_filter_run, src/hash.c:40-839): a device copy of the index
+ * and the ASCII genome is made once on ctx's device; every run uploads the
+ * reads, builds the position lists, pairs them on the device and returns the
+ * same srm_filter (candidates, batch boundaries, perfect hits) as
+ * srm_filter_run on the same arguments.  Runs on ctx's stream; not thread-safe
+ * per object.
+ */
+typedef struct srm_dfilter srm_dfilter;
+int srm_dfilter_create(srm_ctx *ctx, const srm_index *index, const char *genome, size_t glen,
+                       srm_dfilter **out);
+int srm_dfilter_run(srm_dfilter *df, const uint8_t *blocked, const char *reads,
+                    const uint64_t *read_off, const uint16_t *read_len, const uint8_t *read_k,
+                    size_t n_reads, int do_reverse, int chunk_value, srm_filter **out);
+/* last run: device time incl. transfers, lists, list entries, reads that needed HBM slots */
+int srm_dfilter_stats(const srm_dfilter *df, float *device_ms, uint64_t *n_lists,
+                      uint64_t *n_list_entries, uint64_t *n_overflow);
+void srm_dfilter_destroy(srm_dfilter *df);
+
 /* seeded synthetic workloads (xoshiro256**) */
 int srm_synth_genome(uint64_t seed, size_t len, double gc, char *out);
 long long srm_synth_repeats(uint64_t seed, char *genome, size_t len, int n_families, int fam_min,

@@ -28,31 +28,11 @@
 #include <unordered_map>
 #include <vector>
 
-#include "saruman_hip.h"
+#include "filter_internal.h"
+
+using namespace srm_filter_impl;
 
 namespace {
-
-inline int code_of(char c)
-{
-    switch (c) {
-    case 'A': return 0;
-    case 'C': return 1;
-    case 'G': return 2;
-    case 'T': return 3;
-    default: return -1;
-    }
-}
-
-inline char comp(char c)
-{
-    switch (c) {
-    case 'A': return 'T';
-    case 'C': return 'G';
-    case 'G': return 'C';
-    case 'T': return 'A';
-    default: return c;
-    }
-}
 
 int nthreads(int want)
 {
@@ -78,60 +58,6 @@ void parallel_for(size_t n, int T, F f)
     for (auto &x : th) x.join();
 }
 
-struct QIndex {
-    int q = 0;
-    int pos_bits = 0;
-    std::vector<uint64_t> keys;            // (code << pos_bits) | pos, ascending
-    int pre_bases = 0;
-    std::vector<uint32_t> pre;             // offsets by the first pre_bases bases
-    std::unordered_map<std::string, std::vector<uint32_t>> odd;   // q-grams with non-ACGT
-
-    void lookup(const char *g, std::vector<uint32_t> &out) const
-    {
-        uint64_t code = 0;
-        for (int i = 0; i < q; ++i) {
-            int c = code_of(g[i]);
-            if (c < 0) {
-                auto it = odd.find(std::string(g, (size_t)q));
-                if (it != odd.end()) out.insert(out.end(), it->second.begin(), it->second.end());
-                return;
-            }
-            code = (code << 2) | (uint64_t)c;
-        }
-        uint64_t p = code >> (2 * (q - pre_bases));
-        size_t lo = pre[p], hi = pre[p + 1];
-        const uint64_t mask = ((uint64_t)1 << pos_bits) - 1;
-        auto first = std::lower_bound(keys.begin() + (long)lo, keys.begin() + (long)hi,
-                                      code << pos_bits);
-        for (auto it = first; it != keys.begin() + (long)hi && ((*it) >> pos_bits) == code; ++it)
-            out.push_back((uint32_t)(*it & mask));
-    }
-};
-
-struct Params {
-    int L = 0, k = 0, q = 0, rest = 0, n = 0, half = 0, exact_tresh = 0;
-    std::vector<int> gram;   // q-gram offsets in the read
-};
-
-// saruman.c:42-75, :119 (float arithmetic as in the reference)
-Params derive(int L, int k)
-{
-    Params p;
-    p.L = L;
-    p.k = k;
-    p.q = (int)std::ceil((float)L / (float)(k + 1)) - 1;
-    if (p.q < 1) p.q = 1;
-    p.rest = L % p.q;
-    p.n = L / p.q;
-    if (p.rest != 0) p.n *= 2;
-    for (int i = 0; i < L / p.q; ++i) p.gram.push_back(i * p.q);
-    for (int z = 1; z <= L / p.q; ++z) p.gram.push_back(L - z * p.q);
-    p.gram.resize((size_t)p.n);
-    p.half = p.n / 2;
-    p.exact_tresh = (int)std::floor((float)L / (float)p.q) - k + 1;
-    return p;
-}
-
 struct Entry {
     int32_t pos;
     int8_t flag;
@@ -144,27 +70,6 @@ struct ReadOut {
 };
 
 }  // namespace
-
-struct srm_index {
-    std::vector<QIndex> idx;
-    size_t glen = 0;
-    const QIndex *get(int q) const
-    {
-        for (auto &x : idx)
-            if (x.q == q) return &x;
-        return nullptr;
-    }
-};
-
-struct srm_filter {
-    std::vector<int32_t> c_read;
-    std::vector<int64_t> c_pos;
-    std::vector<uint8_t> c_flag;
-    std::vector<uint16_t> c_w;
-    std::vector<uint64_t> batch_end;
-    std::vector<int32_t> p_read;
-    std::vector<int64_t> p_pos;
-};
 
 namespace {
 
@@ -525,85 +430,16 @@ int srm_filter_run(const srm_index *ix, const char *genome, size_t glen, const u
     }
     for (auto &x : th) x.join();
 
-    // batches: cuda_queue_check after every read, then the final flush.
-    // prepare_cuda_alignments walks every read of the result hash in insertion
-    // (= read) order.  Queued positions are negated, perfect ones are negative, so
-    // only entries of reads since the last flush and entries skipped as out of the
-    // genome (never negated, Q10) are positive; every other entry just resets the
-    // s/e state.  `live` marks positive entries; `sticky` lists older reads that
-    // still hold one.
-    srm_filter *F = new srm_filter();
-    std::vector<size_t> next_with(n_reads + 1, n_reads);   // next read with entries
-    for (size_t r = n_reads; r-- > 0;)
-        next_with[r] = (r + 1 < n_reads && !ro[r + 1].ent.empty()) ? r + 1 : next_with[r + 1];
-    std::vector<std::vector<uint8_t>> live(n_reads);
-    std::vector<size_t> sticky;
-    uint64_t queue = 0;
-    size_t pending_lo = 0;
-    auto flush = [&](size_t lo, size_t hi) {
-        std::vector<size_t> order(sticky);
-        for (size_t r = lo; r < hi; ++r)
-            if (!ro[r].ent.empty()) {
-                live[r].resize(ro[r].ent.size());
-                for (size_t i = 0; i < ro[r].ent.size(); ++i) live[r][i] = ro[r].ent[i].pos > 0;
-                order.push_back(r);
-            }
-        int start_offset = 0, end_offset = 0;
-        size_t before = F->c_read.size();
-        size_t prev = (size_t)-1;
-        std::vector<size_t> still;
-        for (size_t r : order) {
-            if (prev != (size_t)-1 && next_with[prev] != r) {
-                start_offset = 0;   // an intervening read's non-positive entry resets
-                end_offset = 0;
-            }
-            prev = r;
-            const int L = read_len[r], k = read_k[r];
-            bool keep = false;
-            for (size_t i = 0; i < ro[r].ent.size(); ++i) {
-                const Entry &en = ro[r].ent[i];
-                if (live[r][i]) {
-                    switch (en.flag) {   // hash.c:938-979
-                    case 1: case 5: end_offset = k; break;
-                    case 2: case 6: start_offset = k; break;
-                    case 3: case 7: start_offset = k; end_offset = k; break;
-                    default: break;
-                    }
-                    const int64_t w = (int64_t)L + start_offset + end_offset;
-                    bool skip = (uint64_t)en.pos + (uint64_t)w > (uint64_t)glen;   // :982
-                    if (!skip && blocked)
-                        for (int64_t j = 0; j < w && !skip; ++j) skip = blocked[en.pos + j] != 0;
-                    if (skip) {            // `continue`: no reset, stays positive (Q10)
-                        keep = true;
-                        continue;
-                    }
-                    F->c_read.push_back((int32_t)r);
-                    F->c_pos.push_back(en.pos);
-                    F->c_flag.push_back((uint8_t)en.flag);
-                    F->c_w.push_back((uint16_t)w);
-                    live[r][i] = 0;        // :1077 negated
-                }
-                end_offset = 0;
-                start_offset = 0;
-            }
-            if (keep) still.push_back(r);
-        }
-        sticky.swap(still);
-        if (F->c_read.size() > before) F->batch_end.push_back(F->c_read.size());
+    struct Acc {
+        const std::vector<ReadOut> &ro;
+        size_t n_ent(size_t r) const { return ro[r].ent.size(); }
+        int32_t pos_at(size_t r, size_t i) const { return ro[r].ent[i].pos; }
+        int flag_at(size_t r, size_t i) const { return ro[r].ent[i].flag; }
+        size_t n_perf(size_t r) const { return ro[r].perfect.size(); }
+        int32_t perf_at(size_t r, size_t i) const { return ro[r].perfect[i]; }
+        int queued(size_t r) const { return ro[r].queued; }
     };
-    for (size_t r = 0; r < n_reads; ++r) {
-        for (int32_t S : ro[r].perfect) {
-            F->p_read.push_back((int32_t)r);
-            F->p_pos.push_back(S);
-        }
-        queue += (uint64_t)ro[r].queued;
-        if (queue > (uint64_t)chunk_value) {
-            flush(pending_lo, r + 1);
-            pending_lo = r + 1;
-            queue = 0;
-        }
-    }
-    if (queue > 0) flush(pending_lo, n_reads);
+    srm_filter *F = batch_pass(Acc{ro}, n_reads, read_len, read_k, glen, blocked, chunk_value);
     *out = F;
     return SRM_OK;
 }

@@ -1,0 +1,349 @@
+// filter_pair.h -- the per-read part of the q-gram filter as host/device code:
+// q-gram lookup in the device copy of the index, run_seashore's pairing
+// (src/hash.c:502-834) and add_start_position (src/hash.c:40-178) over the flat
+// position-list array.  Compiled into the kernels of filter_gpu.hip; the same
+// functions also compile for the host, which lets a CPU build check the
+// kernels' indexing.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "filter_internal.h"
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define SRM_HD __host__ __device__
+#else
+#define SRM_HD
+#endif
+
+namespace srm_pair {
+
+constexpr uint32_t OVERFLOW_MARK = 0xFFFFFFFFu;
+
+struct DQ {                  // one q's index on the device
+    const uint64_t *keys;    // (code << pos_bits) | offset, ascending
+    const uint32_t *pre;     // [4^pre_bases + 1] key ranges by the first bases
+    const uint64_t *odd_hash;   // FNV-1a of the q-gram text, ascending, then offset
+    const uint32_t *odd_pos;
+    uint32_t n_odd;
+    int q, pos_bits, pre_bases;
+};
+
+struct DParam {
+    int L, k, q, rest, n, half, exact;
+    uint32_t gram_off;       // into FArgs::gram
+};
+
+struct FArgs {
+    const DQ *qix;           // [32], indexed by q
+    const DParam *par;
+    const uint16_t *gram;
+    const char *genome;
+    uint64_t glen;
+    const char *reads;
+    const uint64_t *roff;
+    const uint16_t *rlen;
+    const uint16_t *pid;
+    uint32_t n_reads;
+    int dr;                  // 0 forward, 4 reverse
+    uint32_t cv;             // chunk_value
+    const uint32_t *list_base;   // [n_reads + 1] first list of every read
+    uint32_t *list_cnt;          // [n_lists + 1] list length + 1 (terminator)
+    const uint32_t *list_start;  // [n_lists + 1] exclusive sum of list_cnt
+    uint32_t *flat;
+    // pairing
+    const uint32_t *todo;    // reads to pair; null = all reads
+    uint32_t n_todo;
+    int32_t *slot_pos;       // HBM slots (second and later passes)
+    int8_t *slot_flag;
+    int32_t *slot_perf;
+    uint32_t cap, pcap;
+    uint32_t *counters;      // [2] compact entries / perfect hits written so far
+    int32_t *out_pos;
+    int8_t *out_flag;
+    int32_t *out_perf;
+    uint32_t *meta;          // [3 * n_todo]: ent_off, perf_off, n_ent | n_perf << 16
+};
+
+SRM_HD inline char dcomp(char c)
+{
+    switch (c) {
+    case 'A': return 'T';
+    case 'C': return 'G';
+    case 'G': return 'C';
+    case 'T': return 'A';
+    default: return c;
+    }
+}
+
+SRM_HD inline int dcode(char c)
+{
+    switch (c) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+    default: return -1;
+    }
+}
+
+// byte j of the read as the filter sees it (reverse complement for strand 4,
+// util.c:35-57)
+SRM_HD inline char obyte(const char *rd, int L, int dr, int j)
+{
+    return dr ? dcomp(rd[L - 1 - j]) : rd[j];
+}
+
+// FNV-1a, 64 bit
+SRM_HD inline uint64_t fnv_step(uint64_t h, unsigned char c)
+{
+    return (h ^ c) * 1099511628211ull;
+}
+constexpr uint64_t FNV_BASIS = 14695981039346656037ull;
+
+// positions of the q-gram at read offset g, ascending; writes them to out when
+// FILL, returns their number
+template <bool FILL>
+SRM_HD uint32_t lookup(const DQ &X, const char *genome, const char *rd, int L, int dr, int g,
+                       uint32_t *out)
+{
+    const int q = X.q;
+    uint64_t code = 0;
+    bool odd = false;
+    for (int i = 0; i < q; ++i) {
+        const int c = dcode(obyte(rd, L, dr, g + i));
+        if (c < 0) {
+            odd = true;
+            break;
+        }
+        code = (code << 2) | (uint64_t)c;
+    }
+    uint32_t cnt = 0;
+    if (odd) {
+        uint64_t h = FNV_BASIS;
+        for (int i = 0; i < q; ++i) h = fnv_step(h, (unsigned char)obyte(rd, L, dr, g + i));
+        uint32_t lo = 0, hi = X.n_odd;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (X.odd_hash[mid] < h) lo = mid + 1;
+            else hi = mid;
+        }
+        for (uint32_t i = lo; i < X.n_odd && X.odd_hash[i] == h; ++i) {
+            const uint32_t p = X.odd_pos[i];
+            bool same = true;
+            for (int t = 0; t < q && same; ++t) same = genome[p + t] == obyte(rd, L, dr, g + t);
+            if (same) {
+                if (FILL) out[cnt] = p;
+                ++cnt;
+            }
+        }
+        return cnt;
+    }
+    const uint64_t p = code >> (2 * (q - X.pre_bases));
+    uint32_t lo = X.pre[p], hi = X.pre[p + 1];
+    const uint64_t key = code << X.pos_bits;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (X.keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint64_t mask = ((uint64_t)1 << X.pos_bits) - 1;
+    const uint32_t end = X.pre[p + 1];
+    for (uint32_t i = lo; i < end && (X.keys[i] >> X.pos_bits) == code; ++i) {
+        if (FILL) out[cnt] = (uint32_t)(X.keys[i] & mask);
+        ++cnt;
+    }
+    return cnt;
+}
+
+// a read's result-hash entries (pos, flag) and printed perfect hits
+struct Store {
+    int32_t *pos;
+    int8_t *flag;
+    int32_t *perf;
+    uint32_t stride, cap, pcap;
+    uint32_t n = 0, np = 0;
+    bool ovf = false;
+    SRM_HD int32_t &P(uint32_t i) { return pos[i * stride]; }
+    SRM_HD int8_t &F(uint32_t i) { return flag[i * stride]; }
+    SRM_HD int32_t &Q(uint32_t i) { return perf[i * stride]; }
+    SRM_HD void push(int32_t p, int f)
+    {
+        if (n == cap) {
+            ovf = true;
+            return;
+        }
+        P(n) = p;
+        F(n) = (int8_t)f;
+        ++n;
+    }
+};
+
+// add_start_position, hash.c:40-178: 0 first entry, 1 rejected, 2 added or
+// flag lowered.  |pos - e| <= k is the reference's scan of pos + i, i in [-k, k].
+SRM_HD inline int add_start(Store &s, int32_t pos, int flag, int possible, int exact, int k)
+{
+    if (possible != exact) return 1;
+    const int dir = flag > 3 ? 1 : 0;
+    if (s.n == 0) {
+        s.push(pos, flag);
+        return 0;
+    }
+    for (uint32_t e = 0; e < s.n; ++e) {
+        const int32_t ep = s.P(e);
+        const int ef = s.F(e);
+        const int tdir = ef > 3 ? 1 : 0;
+        const int64_t d1 = (int64_t)ep - pos, d2 = (int64_t)ep + pos;
+        if (ef <= flag && ((d1 >= -k && d1 <= k) || (d2 >= -k && d2 <= k))) return 1;
+        if (ep == pos && tdir == dir && ef > flag) {
+            s.F(e) = (int8_t)flag;
+            return 2;
+        } else if ((ep == pos || ep == -pos) && tdir == dir) {
+            return 1;
+        }
+    }
+    s.push(pos, flag);
+    return 2;
+}
+
+SRM_HD inline void seashore(const FArgs &a, uint32_t r, Store &s)
+{
+    const DParam P = a.par[a.pid[r]];
+    const int q = P.q, half = P.half, k = P.k, L = P.L, rest = P.rest, dr = a.dr;
+    const char *rd = a.reads + a.roff[r];
+    const uint32_t lb = a.list_base[r];
+    const uint32_t next_chunk = (r / a.cv + 1) * a.cv;
+    const uint32_t hi_read = next_chunk < a.n_reads ? next_chunk : a.n_reads;
+    const uint64_t flat_end = a.list_start[a.list_base[hi_read]];
+    auto D = [&](int x, uint64_t idx) -> uint32_t {
+        const uint64_t p = (uint64_t)a.list_start[lb + x] + idx;
+        return p < flat_end ? a.flat[p] : 0u;
+    };
+    for (int aa = 0; aa < half; ++aa) {
+        const bool fqm = (aa == 0);
+        const long in_read = (long)aa * q + 1;
+        uint64_t off = 0;
+        uint32_t Pa = D(aa, off);
+        while (Pa > 0) {
+            const long gs = (long)Pa - in_read;
+            if (gs + 1 > 0) {
+                // perfect check (strncpy + strcmp, hash.c:550-583)
+                const long S = gs + 1;
+                if ((uint64_t)S + (uint64_t)L <= a.glen) {
+                    bool perfect = true;
+                    const char *g = a.genome + S;
+                    for (int j = 0; j < L && perfect; ++j) perfect = g[j] == obyte(rd, L, dr, j);
+                    if (perfect && add_start(s, (int32_t)(-S), 0, P.exact, P.exact, k) != 1) {
+                        if (s.np == s.pcap) s.ovf = true;
+                        else s.Q(s.np++) = (int32_t)S;
+                    }
+                    if (s.ovf) return;
+                }
+                int possible = 1;
+                // first pass, hash.c:606-692
+                for (int b = aa + 1; b < half; ++b) {
+                    const long rstart = (long)b * q + 1;
+                    for (int err = -k; err <= k; ++err) {
+                        uint32_t Pb = D(b, off);
+                        uint64_t nxt = 0;
+                        while (Pb > 0) {
+                            if ((long)Pb == gs + rstart + err) {
+                                int flag;
+                                long pos;
+                                if (b == half - 1 && fqm) {
+                                    flag = 1 + dr;
+                                    pos = gs + 1;
+                                } else if (aa == half - 1) {
+                                    flag = ((long)Pa + q > L ? 3 : 2) + dr;
+                                    pos = gs - k + 1;
+                                } else if (fqm) {
+                                    flag = 1 + dr;
+                                    pos = gs + 1;
+                                } else {
+                                    flag = 3 + dr;
+                                    pos = gs - k + 1;
+                                }
+                                ++possible;
+                                add_start(s, (int32_t)pos, flag, possible, P.exact, k);
+                                if (s.ovf) return;
+                            }
+                            ++nxt;
+                            Pb = D(b, nxt);
+                        }
+                    }
+                }
+                // second pass, hash.c:695-794
+                for (int j = 0; j < half; ++j) {
+                    const int lim = half - 1 - j - aa < 0 ? -(half - 1 - j - aa) : half - 1 - j - aa;
+                    const long nin = (long)(half - j - 1) * q + 1 + rest;
+                    for (int err = -lim; err <= lim; ++err) {
+                        const int aerr = (j - aa < 0) ? -err : err;
+                        uint64_t so = 0;
+                        uint32_t Pb = D(half + j, so);
+                        while (Pb > 0) {
+                            if ((long)Pb == gs + nin + aerr) {
+                                int flag;
+                                long pos;
+                                if (j == 0 && fqm) {
+                                    flag = 1 + dr;
+                                    pos = gs + 1;
+                                } else if (j == 0) {
+                                    flag = (nin + q > L ? 3 : 2) + dr;
+                                    pos = gs - k + 1;
+                                } else if (fqm) {
+                                    flag = 1 + dr;
+                                    pos = gs + 1;
+                                } else {
+                                    flag = 3 + dr;
+                                    pos = gs - k + 1;
+                                }
+                                ++possible;
+                                add_start(s, (int32_t)pos, flag, possible, P.exact, k);
+                                if (s.ovf) return;
+                            }
+                            ++so;
+                            Pb = D(half + j, so);
+                        }
+                    }
+                }
+            }
+            ++off;
+            Pa = D(aa, off);
+        }
+    }
+}
+
+// the device copies of the per-read parameter sets and their q-gram offsets
+inline void device_params(const std::vector<srm_filter_impl::Params> &params,
+                          std::vector<DParam> &dpar, std::vector<uint16_t> &gram)
+{
+    for (auto &p : params) {
+        dpar.push_back({p.L, p.k, p.q, p.rest, p.n, p.half, p.exact_tresh, (uint32_t)gram.size()});
+        for (int g : p.gram) gram.push_back((uint16_t)g);
+    }
+}
+
+// the device copy of one q's index: the odd (non-ACGT) q-grams as (FNV-1a of
+// the text, offset) pairs in ascending order
+inline void odd_table(const srm_filter_impl::QIndex &X, std::vector<uint64_t> &hash,
+                      std::vector<uint32_t> &pos)
+{
+    std::vector<std::pair<uint64_t, uint32_t>> odd;
+    for (auto &kv : X.odd) {
+        uint64_t h = FNV_BASIS;
+        for (char c : kv.first) h = fnv_step(h, (unsigned char)c);
+        for (uint32_t p : kv.second) odd.emplace_back(h, p);
+    }
+    std::sort(odd.begin(), odd.end());
+    hash.resize(odd.size());
+    pos.resize(odd.size());
+    for (size_t t = 0; t < odd.size(); ++t) {
+        hash[t] = odd[t].first;
+        pos[t] = odd[t].second;
+    }
+}
+
+}  // namespace srm_pair

@@ -52,4 +52,9 @@ hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 
+// runtime services for the other device modules (defined in nw_runtime.hip)
+int set_error(int code, const char *fmt, ...);   // sets srm_last_error(), returns code
+int ctx_device(const srm_ctx *c);
+hipStream_t ctx_stream(const srm_ctx *c);
+
 }  // namespace srm

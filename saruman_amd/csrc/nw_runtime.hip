@@ -302,6 +302,25 @@ int run_launches(srm_job *j, int kernel, bool timed)
 
 }  // namespace
 
+namespace srm {
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int ctx_device(const srm_ctx *c) { return c->device; }
+
+hipStream_t ctx_stream(const srm_ctx *c) { return c->stream; }
+
+}  // namespace srm
+
 // ---------------------------------------------------------------------------
 // extended API
 // ---------------------------------------------------------------------------
@@ -636,8 +655,9 @@ int srm_job_run(srm_job *j, int kernel)
     HIP_TRY(hipEventElapsedTime(&j->band_ms, j->ev[0], j->ev[1]));
     HIP_TRY(hipEventElapsedTime(&j->plain_ms, j->ev[1], j->ev[2]));
     for (auto &x : j->class_ms) x = 0.f;
-    for (size_t c = 0; c < j->classes.size(); ++c)
-        HIP_TRY(hipEventElapsedTime(&j->class_ms[j->classes[c].nw], j->cev[c], j->cev[c + 1]));
+    if (kernel != SRM_KERNEL_PLAIN)   // the class events are recorded only with band launches
+        for (size_t c = 0; c < j->classes.size(); ++c)
+            HIP_TRY(hipEventElapsedTime(&j->class_ms[j->classes[c].nw], j->cev[c], j->cev[c + 1]));
     return SRM_OK;
 }
 

@@ -158,21 +158,15 @@ def needed_qs(reads: Reads):
     return sorted({filter_params(L, k)[0] for L, k in pairs})
 
 
-def run_filter(index: Index, genome: Genome, reads: Reads, do_reverse: int, chunk_value: int,
-               threads: int = 0) -> StrandCands:
-    import time
-    L = index.lib
-    h = _vp()
-    g = np.frombuffer(genome.seq, np.uint8)
+def _reads_args(reads: Reads):
     rs = np.frombuffer(reads.seq, np.uint8)
     off = np.ascontiguousarray(reads.off, np.uint64)
     ln = np.ascontiguousarray(reads.length, np.uint16)
     kk = np.ascontiguousarray(reads.k, np.uint8)
-    bl = genome.blocked
-    t0 = time.perf_counter()
-    check(L.srm_filter_run(index.h, ptr(g), len(g), ptr(bl), ptr(rs), ptr(off), ptr(ln), ptr(kk),
-                           reads.n, do_reverse, chunk_value, threads, C.byref(h)), "srm_filter_run")
-    secs = time.perf_counter() - t0
+    return rs, off, ln, kk
+
+
+def _collect(L, h, do_reverse: int, secs: float) -> StrandCands:
     nc, nb, npf = C.c_uint64(), C.c_uint64(), C.c_uint64()
     L.srm_filter_sizes(h, C.byref(nc), C.byref(nb), C.byref(npf))
     n, b, p = nc.value, nb.value, npf.value
@@ -183,6 +177,63 @@ def run_filter(index: Index, genome: Genome, reads: Reads, do_reverse: int, chun
                       ptr(out.batch_end), ptr(out.perfect_read), ptr(out.perfect_pos))
     L.srm_filter_destroy(h)
     return out
+
+
+def run_filter(index: Index, genome: Genome, reads: Reads, do_reverse: int, chunk_value: int,
+               threads: int = 0) -> StrandCands:
+    """Host filter (srm_filter_run) of one strand."""
+    import time
+    L = index.lib
+    h = _vp()
+    g = np.frombuffer(genome.seq, np.uint8)
+    rs, off, ln, kk = _reads_args(reads)
+    t0 = time.perf_counter()
+    check(L.srm_filter_run(index.h, ptr(g), len(g), ptr(genome.blocked), ptr(rs), ptr(off),
+                           ptr(ln), ptr(kk), reads.n, do_reverse, chunk_value, threads,
+                           C.byref(h)), "srm_filter_run")
+    return _collect(L, h, do_reverse, time.perf_counter() - t0)
+
+
+class DeviceFilter:
+    """The same filter with its per-read work on the GPU (srm_dfilter_*): the
+    index and the genome are copied to ctx's device once."""
+
+    def __init__(self, ctx, index: Index, genome: Genome):
+        self.lib = index.lib
+        self.index, self.genome = index, genome     # kept alive: the object borrows them
+        g = np.frombuffer(genome.seq, np.uint8)
+        h = _vp()
+        check(self.lib.srm_dfilter_create(ctx.h, index.h, ptr(g), len(g), C.byref(h)),
+              "srm_dfilter_create")
+        self.h = h
+
+    def run(self, reads: Reads, do_reverse: int, chunk_value: int) -> StrandCands:
+        import time
+        h = _vp()
+        rs, off, ln, kk = _reads_args(reads)
+        t0 = time.perf_counter()
+        check(self.lib.srm_dfilter_run(self.h, ptr(self.genome.blocked), ptr(rs), ptr(off),
+                                       ptr(ln), ptr(kk), reads.n, do_reverse, chunk_value,
+                                       C.byref(h)), "srm_dfilter_run")
+        return _collect(self.lib, h, do_reverse, time.perf_counter() - t0)
+
+    def stats(self) -> dict:
+        ms = C.c_float()
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.lib.srm_dfilter_stats(self.h, C.byref(ms), C.byref(a), C.byref(b), C.byref(c))
+        return {"device_ms": ms.value, "lists": a.value, "list_entries": b.value,
+                "overflow_reads": c.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.srm_dfilter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _COMP = np.zeros(256, np.uint8)
