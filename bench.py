@@ -29,6 +29,34 @@ PEAK_INT32_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz, int32 VALU ops/s
 PEAK_HBM_GBS = 8000.0
 OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 
+# HBM bytes of one step of the band kernels, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (scaled x2.000 and x1.000 by a 256 MiB copy kernel), for the exact
+# configuration profiled: workload, candidates, launch tuning and the sha256
+# prefix of the kernel source.  Source: profiles/band_kernel_mixed_2M_reads.md
+PMC_TRAFFIC = [
+    {"workload": "mixed", "candidates": 1774238,
+     "tuning": {"blocks_per_cu": [0, 2, 0, 3, 0, 1, 1, 2, 0], "merge_below": 3},
+     "kernel_sha": "09fee5885613", "bytes": 12005253835},
+]
+
+
+def kernel_sha() -> str:
+    import hashlib
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "saruman_amd", "csrc",
+                       "nw_kernels.hip")
+    with open(src, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:12]
+
+
+def profiled_traffic(workload, candidates, tuning):
+    tun = ({k: tuning[k] for k in ("blocks_per_cu", "merge_below")} if tuning else "defaults")
+    sha = kernel_sha()
+    for e in PMC_TRAFFIC:
+        if (e["workload"] == workload and e["candidates"] == candidates and e["tuning"] == tun
+                and e["kernel_sha"] == sha):
+            return e["bytes"]
+    return None
+
 WORKLOAD_TEXT = {
     "small": "1 Mbp uniform genome, 100,000 x 36 bp reads, 0-2 errors, k=2, both strands",
     "single": "5 Mbp 50% GC genome, 1,000,000 x 100 bp reads (sub 1%, indel 0.1%, 5% random), "
@@ -117,6 +145,33 @@ def serve_leg(args):
     return out
 
 
+def device_filter_leg(ctx, index, genome, reads, chunk, host_strands):
+    """Both strands through the GPU q-gram filter (one untimed pass, one timed),
+    checked field by field against the host filter's output."""
+    from saruman_amd import workload as wl
+    df = wl.DeviceFilter(ctx, index, genome)
+    try:
+        for dr in (0, 4):
+            df.run(reads, dr, chunk)
+        t0 = time.perf_counter()
+        out, ms, st = [], 0.0, []
+        for dr in (0, 4):
+            out.append(df.run(reads, dr, chunk))
+            s = df.stats()
+            ms += s["device_ms"]
+            st.append(s)
+        wall = time.perf_counter() - t0
+    finally:
+        df.close()
+    fields = ("read", "pos", "flag", "w", "batch_end", "perfect_read", "perfect_pos")
+    same = all(np.array_equal(getattr(h, f), getattr(d, f))
+               for h, d in zip(host_strands, out) for f in fields)
+    return {"wall_s": wall, "device_ms": ms, "identical": bool(same),
+            "host_s": sum(h.seconds for h in host_strands),
+            "lists": sum(s["lists"] for s in st), "list_entries": sum(s["list_entries"] for s in st),
+            "overflow_reads": sum(s["overflow_reads"] for s in st)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +186,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tuning", action="store_true", help="ignore saruman_amd/tuning/*.json")
+    ap.add_argument("--no-dfilter", action="store_true", help="skip the GPU q-gram filter leg")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -154,7 +210,6 @@ def main():
     t0 = time.perf_counter()
     strands = [wl.run_filter(index, genome, reads, dr, cfg["chunk"]) for dr in (0, 4)]
     t_filter = time.perf_counter() - t0
-    index.close()
     db = wl.device_batch(reads, strands)
     perfect_reads = np.unique(np.concatenate([s.perfect_read for s in strands]))
     n_batches = sum(len(s.batch_end) for s in strands)
@@ -162,6 +217,13 @@ def main():
         f"{n_batches} reference batches, {len(perfect_reads)} reads with perfect hits")
 
     ctx = sa.Context(0)
+    gpu_filter = None
+    if not args.no_dfilter:
+        gpu_filter = device_filter_leg(ctx, index, genome, reads, cfg["chunk"], strands)
+        log(f"[bench] GPU filter: {gpu_filter['wall_s']:.3f}s for both strands "
+            f"(device {gpu_filter['device_ms']:.1f} ms), identical to the host filter: "
+            f"{gpu_filter['identical']}")
+    index.close()
     from saruman_amd import tune
     tuning = None if args.no_tuning else tune.apply_cached(ctx)
     log(f"[bench] launch tuning: {tuning if tuning else 'defaults'}")
@@ -261,7 +323,8 @@ def main():
         "gcups": ref_cells / (ms_per_step / 1e3) / 1e9,
         "band_gcups": bcells / (band_ms / 1e3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
-                     "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS, "traffic": None,
+                     "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS,
+                     "traffic": profiled_traffic(args.workload, int(db.n), tuning),
                      "kernel": "band_kernel (all band classes of one step)",
                      "ops_per_step": OPS_PER_BAND_CELL * bcells,
                      "kernel_ms": band_ms},
@@ -273,6 +336,7 @@ def main():
         "host": {"generate_s": t_gen, "index_s": t_index, "filter_s": t_filter,
                  "upload_s": t_upload, "download_s": t_d2h},
         "e2e_reads_per_s_pcie": n_mapped / (t_upload + ms_per_step / 1e3 + t_d2h),
+        "gpu_filter": gpu_filter,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

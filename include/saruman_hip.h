@@ -246,6 +246,22 @@ int srm_job_use_graph(srm_job *job, int enable);
 int srm_job_path_count(srm_job *job, int path);
 
 /*
+ * TSV records of every hit of the job's last run, formatted on the device in
+ * candidate order: the lines srm_format_records writes on the host, i.e. the
+ * post-processing and printf of src/cuda-functions.cu:828-994 for each candidate
+ * with score <= k.  pos[n]: the window's genome position (record start before
+ * leading gaps).  Names: with a table, names[read_id[t] * name_stride ...] (NUL
+ * terminated within the stride); without, name_prefix + decimal read_id[t]
+ * (read_id NULL: the candidate index).  Returns the text's byte count; the text
+ * is written to out only when outcap >= that count (a size query otherwise), or
+ * an SRM_E* code (< 0).  n_hits (may be NULL) receives the number of records.
+ * Synchronous on the job's stream.
+ */
+long long srm_job_records(srm_job *job, const int64_t *pos, const int32_t *read_id,
+                          const char *names, int name_stride, size_t n_names,
+                          const char *name_prefix, char *out, size_t outcap, long long *n_hits);
+
+/*
  * Record text for one hit: the post-processing of src/cuda-functions.cu:828-994
  * applied to a candidate whose slot rows are reconstructed from its transcript.
  * read/win are the candidate's bytes (read oriented as aligned).  Writes one

@@ -49,6 +49,8 @@ _PROTOS = [
     ("srm_job_kernel_ms", C.c_float, [_vp, C.c_int]),
     ("srm_job_use_graph", C.c_int, [_vp, C.c_int]),
     ("srm_job_path_count", C.c_int, [_vp, C.c_int]),
+    ("srm_job_records", C.c_longlong, [_vp, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_char_p, _vp,
+                                       C.c_size_t, _vp]),
     ("srm_format_record", C.c_int, [C.c_char_p, C.c_uint, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _vp, _vp, _vp, _vp, C.c_char_p, C.c_int]),
     ("srm_slot_rows", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),

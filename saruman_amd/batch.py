@@ -190,6 +190,31 @@ class Job:
         check(self.lib.srm_job_download(self.h, ptr(res), ptr(ops), ops.size), "srm_job_download")
         return res, ops
 
+    def records(self, pos, read_id=None, names=None, name_stride=60, prefix="", out=None):
+        """TSV records of every hit of the last run, formatted on the device
+        (srm_job_records).  names: None (prefix + decimal read id) or a bytes
+        table of name_stride-byte NUL-terminated names indexed by read_id.
+        out: optional preallocated uint8 buffer.  Returns (text bytes, hits)."""
+        pos = np.ascontiguousarray(pos, np.int64)
+        rid = None if read_id is None else np.ascontiguousarray(read_id, np.int32)
+        nb = None if names is None else np.frombuffer(names, np.uint8)
+        n_names = 0 if nb is None else nb.size // name_stride
+        pfx = prefix.encode()
+        hits = C.c_longlong()
+        if out is None:
+            out = np.empty(1, np.uint8)
+        got = self.lib.srm_job_records(self.h, ptr(pos), ptr(rid), ptr(nb), name_stride, n_names,
+                                       pfx, ptr(out), out.size, C.byref(hits))
+        if got < 0:
+            check(int(got), "srm_job_records")
+        if got > out.size:          # the first call was a size query
+            out = np.empty(int(got), np.uint8)
+            got = self.lib.srm_job_records(self.h, ptr(pos), ptr(rid), ptr(nb), name_stride,
+                                           n_names, pfx, ptr(out), out.size, C.byref(hits))
+            if got < 0:
+                check(int(got), "srm_job_records")
+        return out[:got].tobytes(), int(hits.value)
+
     def close(self):
         if self.h:
             self.lib.srm_job_destroy(self.h)

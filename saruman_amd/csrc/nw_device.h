@@ -46,7 +46,33 @@ struct BandArgs {
     uint32_t *ops;
 };
 
+struct RecArgs {
+    int n;
+    const srm_cand *cand;
+    const srm_result *res;
+    const uint32_t *ops;
+    const uint32_t *read_words;     // unpadded origin (2-bit path)
+    const uint32_t *win_words;
+    const char *read_bytes;         // raw bytes (SRM_CAND_BYTES candidates)
+    const char *win_bytes;
+    const int64_t *pos;             // window genome position per candidate
+    const int32_t *read_id;         // name index per candidate (null: candidate index)
+    const char *names;              // name table, name_stride bytes per name (or null)
+    int name_stride;
+    const char *prefix;             // name prefix when there is no table
+    int prefix_len;
+    uint64_t *len;                  // [n + 1] line lengths (0: no record)
+    uint4 *info;                    // [n] start, stop, lead | chars << 16
+    uint32_t *hits;                 // records counted
+    const uint64_t *off;            // [n + 1] line offsets
+    char *out;
+};
+
 hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st);
+hipError_t launch_rec_measure(const RecArgs &a, hipStream_t st);
+hipError_t launch_rec_write(const RecArgs &a, hipStream_t st);
+hipError_t rec_scan(void *tmp, size_t &tmp_bytes, const uint64_t *len, uint64_t *off, int n,
+                    hipStream_t st);
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);

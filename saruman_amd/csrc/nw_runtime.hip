@@ -148,6 +148,8 @@ struct srm_job {
     DevBuf d_idx_rest, d_idx_all, d_idx_bytes;      // plain index lists
     DevBuf d_off_rest, d_off_all, d_off_bytes;      // matrix offsets (per list, chunk-relative)
     DevBuf d_mat, d_res, d_ops;
+    DevBuf d_rec_len, d_rec_off, d_rec_info, d_rec_tmp, d_rec_out, d_rec_pos, d_rec_id,
+        d_rec_names, d_rec_hits;                    // device record formatting
     size_t ops_words = 0;
     // host-side plan
     std::vector<BandClass> classes;
@@ -460,7 +462,10 @@ void srm_job_destroy(srm_job *j)
     DevBuf *bufs[] = {&j->d_cand, &j->d_read_words, &j->d_win_words, &j->d_read_bytes,
                       &j->d_win_bytes, &j->d_perm, &j->d_tiles, &j->d_queue, &j->d_scratch,
                       &j->d_idx_rest, &j->d_idx_all, &j->d_idx_bytes, &j->d_off_rest,
-                      &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops};
+                      &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops,
+                      &j->d_rec_len, &j->d_rec_off, &j->d_rec_info, &j->d_rec_tmp,
+                      &j->d_rec_out, &j->d_rec_pos, &j->d_rec_id, &j->d_rec_names,
+                      &j->d_rec_hits};
     for (DevBuf *b : bufs) c->release(*b);
     for (auto &e : j->ev)
         if (e) (void)hipEventDestroy(e);
@@ -674,6 +679,84 @@ int srm_job_download(srm_job *j, srm_result *res, uint32_t *ops, size_t n_ops_wo
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SRM_OK;
+}
+
+long long srm_job_records(srm_job *j, const int64_t *pos, const int32_t *read_id,
+                          const char *names, int name_stride, size_t n_names,
+                          const char *name_prefix, char *out, size_t outcap, long long *n_hits)
+{
+    if (!j->uploaded) return fail(SRM_ESTATE, "srm_job_records before a successful upload");
+    if (!pos) return fail(SRM_EINVAL, "srm_job_records needs the window positions");
+    srm_ctx *ctx = j->ctx;
+    const int n = j->n;
+    if (names) {
+        if (name_stride < 2 || !read_id)
+            return fail(SRM_EINVAL, "a name table needs name_stride >= 2 and read ids");
+        for (int t = 0; t < n; ++t)
+            if (read_id[t] < 0 || (size_t)read_id[t] >= n_names)
+                return fail(SRM_EINVAL, "read id %d of candidate %d is outside the name table",
+                            read_id[t], t);
+    } else if (read_id) {
+        for (int t = 0; t < n; ++t)
+            if (read_id[t] < 0) return fail(SRM_EINVAL, "negative read id at candidate %d", t);
+    }
+    const char *prefix = name_prefix ? name_prefix : "";
+    const size_t plen = strlen(prefix);
+    if (plen > 255) return fail(SRM_EINVAL, "name prefix longer than 255 bytes");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = upload(ctx, j->d_rec_pos, pos, (size_t)n))) return rc;
+    if (read_id && (rc = upload(ctx, j->d_rec_id, read_id, (size_t)n))) return rc;
+    if (names) {
+        if ((rc = upload(ctx, j->d_rec_names, names, n_names * (size_t)name_stride))) return rc;
+    } else if ((rc = upload(ctx, j->d_rec_names, prefix, plen + 1))) {
+        return rc;
+    }
+    if ((rc = ctx->ensure(j->d_rec_len, ((size_t)n + 1) * 8))) return rc;
+    if ((rc = ctx->ensure(j->d_rec_off, ((size_t)n + 1) * 8))) return rc;
+    if ((rc = ctx->ensure(j->d_rec_info, std::max<size_t>((size_t)n * 16, 16)))) return rc;
+    if ((rc = ctx->ensure(j->d_rec_hits, 256))) return rc;
+    srm::RecArgs a{};
+    a.n = n;
+    a.cand = (const srm_cand *)j->d_cand.p;
+    a.res = (const srm_result *)j->d_res.p;
+    a.ops = (const uint32_t *)j->d_ops.p;
+    a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
+    a.win_words = j->resident_genome ? ctx->genome_ptr()
+                                     : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
+    a.read_bytes = (const char *)j->d_read_bytes.p;
+    a.win_bytes = (const char *)j->d_win_bytes.p;
+    a.pos = (const int64_t *)j->d_rec_pos.p;
+    a.read_id = read_id ? (const int32_t *)j->d_rec_id.p : nullptr;
+    a.names = names ? (const char *)j->d_rec_names.p : nullptr;
+    a.name_stride = name_stride;
+    a.prefix = names ? nullptr : (const char *)j->d_rec_names.p;
+    a.prefix_len = (int)plen;
+    a.len = (uint64_t *)j->d_rec_len.p;
+    a.info = (uint4 *)j->d_rec_info.p;
+    a.hits = (uint32_t *)j->d_rec_hits.p;
+    a.off = (const uint64_t *)j->d_rec_off.p;
+    HIP_TRY(hipMemsetAsync(a.hits, 0, 4, ctx->stream));
+    HIP_TRY(srm::launch_rec_measure(a, ctx->stream));
+    size_t tmp = 0;
+    HIP_TRY(srm::rec_scan(nullptr, tmp, a.len, (uint64_t *)j->d_rec_off.p, n, ctx->stream));
+    if ((rc = ctx->ensure(j->d_rec_tmp, std::max<size_t>(tmp, 256)))) return rc;
+    HIP_TRY(srm::rec_scan(j->d_rec_tmp.p, tmp, a.len, (uint64_t *)j->d_rec_off.p, n, ctx->stream));
+    uint64_t total = 0;
+    uint32_t hits = 0;
+    HIP_TRY(hipMemcpyAsync(&total, (const uint64_t *)j->d_rec_off.p + n, 8, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&hits, a.hits, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n_hits) *n_hits = hits;
+    if (total > outcap || !out) return (long long)total;   // size query
+    if ((rc = ctx->ensure(j->d_rec_out, std::max<size_t>(total, 256)))) return rc;
+    a.out = (char *)j->d_rec_out.p;
+    HIP_TRY(srm::launch_rec_write(a, ctx->stream));
+    if (total)
+        HIP_TRY(hipMemcpyAsync(out, a.out, total, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return (long long)total;
 }
 
 float srm_job_last_ms(srm_job *j) { return j->last_ms; }

@@ -118,6 +118,38 @@ def test_edges(orc):
             check(batch, sa.align_batch(batch, kernel=kernel), want, kernel)
 
 
+# (match, mismatch, k, window, read, flag, pos, record) worked by hand; the same
+# cases pin the oracle in tests/test_oracle.py
+HAND = [
+    (0, 1, 2, b"ACGTAA", b"ACGT", 1, 1000, "r0\t1000\t1003\t>>\tACGT\tACGT\t0\n"),
+    (0, 1, 2, b"ACGTAA", b"ACTT", 5, 7, "r1\t7\t10\t<<\tACTT\tACGT\t1\n"),
+    (0, 1, 1, b"ACGTA", b"CGT", 3, 50, "r2\t51\t53\t>>\tCGT\tCGT\t0\n"),        # Q4
+    (0, 1, 1, b"ACGAA", b"TACG", 1, 10, "r3\t10\t12\t>>\tTACG\t_ACG\t1\n"),     # row 0
+    (0, 1, 2, b"CCCCCGATACAC", b"GATTACAC", 3, 200,
+     "r4\t200\t206\t>>\tGATTACAC\tGA_TACAC\t2\n"),                          # Q5
+]
+
+
+@pytest.mark.parametrize("kernel", ["banded", "plain"])
+def test_hand_worked_records(kernel):
+    """Both kernels reproduce the hand-worked records (one launch, each case
+    behind a filler candidate so none is the first of the launch)."""
+    banner(f"[gpu] hand-worked records {kernel}")
+    from saruman_amd.batch import Batch
+    win, rd, fl, ps, nm, ks = [], [], [], [], [], []
+    for i, (ma, mm, k, w, r, f, p, _) in enumerate(HAND):
+        win += [b"ACGTACGTAC", w]
+        rd += [b"ACGTACGT", r]
+        fl += [1, f]
+        ps += [1, p]
+        nm += [f"f{i}", f"r{i}"]
+        ks += [1, k]
+    batch = Batch(0, 1, 2, win, rd, np.array(fl, np.uint8), np.array(ps, np.int64), nm,
+                  ks=np.array(ks, np.int64))
+    got = sa.align_batch(batch, kernel=kernel)
+    assert [got.records[2 * i + 1] for i in range(len(HAND))] == [h[7] for h in HAND]
+
+
 def test_graph_replay(orc):
     """Replays of the captured launch sequence, with eager work and syncs between
     them, each checked against the eager result and the oracle."""

@@ -218,6 +218,24 @@ def test_hand_row0_insertion_and_q1(orc):
     assert orc.record("t", 10, 1, 1, 4, 5, 1, r0.R, r0.G) == "t\t10\t13\t>>\tACG_\tACGA\t1\n"
 
 
+def test_hand_q5_column0_stop(orc):
+    """Q5, worked by hand (cuda-functions.cu:181-190, 239-276).  Flag 3, k = 2, so
+    s = e = 2 and column 0 holds max(0, h - 4).  The read GATTACAC sits at window
+    offset 5 with its fourth base inserted: score 1 (column 0, row 5) + 1 = 2.
+    Walk from (12, 8): five diagonal matches down to (8, 4), the left step of the
+    inserted T at (7, 3) (top gives 3 + 1 != 2), diagonals to (6, 1) and (5, 0).
+    At (5, 0): diag/left read 999, top = 0, and the start clause (column <= 1,
+    row - 1 = 4 <= 2(s + 1)) makes the top cost 0, so 0 != 1 and the walk stops.
+    The record then reports the window start (200), not the alignment start 205,
+    and stop = 8 chars - 1 insertion + 200 - 1 = 206."""
+    r = orc.align_one(0, 1, 2, b"CCCCCGATACAC", b"GATTACAC", 3)
+    assert r.score == 2
+    assert r.R == b"GATTACAC" + b"\0" * 6 and r.G == b"GA_TACAC" + b"\0" * 6
+    assert (r.end_row, r.end_col) == (5, 0)
+    rec = orc.record("z", 200, 3, 2, 8, 12, r.score, r.R, r.G)
+    assert rec == "z\t200\t206\t>>\tGATTACAC\tGA_TACAC\t2\n"
+
+
 def test_record_end_of_read_branch(orc):
     """The eor > 0 branch (cuda-functions.cu:974-976), worked by hand on a slot:
     w = 6, k = 1, R = "AC_G" then NUL: the scan bound w - stc - sc + z exceeds
