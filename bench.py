@@ -35,7 +35,8 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 # prefix of the kernel source.  Source: profiles/band_kernel_mixed_2M_reads.md
 PMC_TRAFFIC = [
     {"workload": "mixed", "candidates": 1774238,
-     "tuning": {"blocks_per_cu": [0, 2, 0, 3, 0, 1, 1, 2, 0], "merge_below": 3},
+     "tuning": {"blocks_per_cu": [0, 2, 0, 3, 0, 1, 1, 2, 0], "merge_below": 3,
+                "class_streams": 0},
      "kernel_sha": "09fee5885613", "bytes": 12005253835},
 ]
 
@@ -48,8 +49,16 @@ def kernel_sha() -> str:
         return hashlib.sha256(f.read()).hexdigest()[:12]
 
 
+def tuning_key(tuning):
+    """The launch tuning as reported: "defaults" or {blocks_per_cu, merge_below}."""
+    if not tuning or (not any(tuning["blocks_per_cu"]) and tuning.get("merge_below", 0) <= 1
+                      and tuning.get("class_streams", 0) == 0):
+        return "defaults"
+    return {k: tuning.get(k, 0) for k in ("blocks_per_cu", "merge_below", "class_streams")}
+
+
 def profiled_traffic(workload, candidates, tuning):
-    tun = ({k: tuning[k] for k in ("blocks_per_cu", "merge_below")} if tuning else "defaults")
+    tun = tuning_key(tuning)
     sha = kernel_sha()
     for e in PMC_TRAFFIC:
         if (e["workload"] == workload and e["candidates"] == candidates and e["tuning"] == tun
@@ -118,31 +127,77 @@ def cpu_baseline(genome, reads, db, n_sample, mapped_per_cand):
 
 def serve_leg(args):
     """Continuous batching over the resident 5 Mbp genome: Poisson arrivals, request
-    sizes log-uniform in 1,000-200,000 reads of 36-250 bp, k = ceil(5% L)."""
+    sizes log-uniform in 1,000-200,000 reads of 36-250 bp, k = ceil(5% L).  The
+    filter and the record formatting run on the device (the host at the first
+    rate as well, for comparison); arrival rates are swept."""
     from saruman_amd import serve
     from saruman_amd import workload as wl
     cfg, genome, pool = wl.make_workload("serve", n_reads=400_000)
     rng = np.random.default_rng(cfg["seed"])
     sizes = serve.request_sizes(rng, args.serve_requests)
     qs = serve.qs_for_lengths(36, 250)
-    with serve.Server(genome, qs, n_slots=args.serve_slots, chunk_value=cfg["chunk"]) as S:
-        serve.poisson_load(S, pool, 50.0, [2000, 2000], seed=1)     # warm the slots
-        res = serve.poisson_load(S, pool, args.serve_rate, sizes, seed=cfg["seed"])
-    st = res.stats
-    dev = sorted((s["t_admitted"], s["t_aligned"]) for s in st)
-    overlapped = sum(1 for j in range(1, len(dev)) if dev[j][0] < dev[j - 1][1])
-    out = {"workload": "5 Mbp resident genome; Poisson arrivals; request sizes log-uniform in "
-                       "1,000-200,000 reads of 36-250 bp; k=ceil(5% L)",
-           "requests": len(st), "rate_hz": args.serve_rate, "slots": args.serve_slots,
-           "p50_ms": res.p50 * 1e3, "p99_ms": res.p99 * 1e3,
-           "reads_per_s": res.reads / res.wall, "mapped_reads_per_s": res.mapped / res.wall,
-           "reads": res.reads, "wall_s": res.wall,
-           "device_intervals_overlapping": overlapped,
-           "mean_filter_ms": float(np.mean([(s["t_filtered"] - s["t_submit"]) for s in st])) * 1e3,
-           "mean_device_ms": float(np.mean([s["device_ms"] for s in st]))}
-    log(f"[bench] serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
-        f"{out['reads_per_s']:.0f} reads/s over {len(st)} requests")
-    return out
+    rates = [float(x) for x in args.serve_rates.split(",")]
+
+    def one(rate, device):
+        with serve.Server(genome, qs, n_slots=args.serve_slots, chunk_value=cfg["chunk"],
+                          gpu_filter=device, gpu_records=device) as S:
+            serve.poisson_load(S, pool, 50.0, [2000, 2000], seed=1)     # warm the slots
+            res = serve.poisson_load(S, pool, rate, sizes, seed=cfg["seed"])
+        st = res.stats
+        dev = sorted((s["t_admitted"], s["t_aligned"]) for s in st)
+        overlapped = sum(1 for j in range(1, len(dev)) if dev[j][0] < dev[j - 1][1])
+        out = {"rate_hz": rate, "filter_and_records": "device" if device else "host",
+               "p50_ms": res.p50 * 1e3, "p99_ms": res.p99 * 1e3,
+               "reads_per_s": res.reads / res.wall, "mapped_reads_per_s": res.mapped / res.wall,
+               "reads": res.reads, "wall_s": res.wall,
+               "device_intervals_overlapping": overlapped,
+               "mean_filter_ms": float(np.mean([(s["t_filtered"] - s["t_submit"]) for s in st])) * 1e3,
+               "mean_device_ms": float(np.mean([s["device_ms"] for s in st]))}
+        log(f"[bench] serving ({out['filter_and_records']} filter/records) at {rate:g}/s: "
+            f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
+            f"{out['reads_per_s']:.0f} reads/s over {len(st)} requests")
+        return out
+
+    runs = [one(r, True) for r in rates]
+    if not args.serve_device_only:
+        runs.append(one(rates[0], False))
+    return {"workload": "5 Mbp resident genome; Poisson arrivals; request sizes log-uniform in "
+                        "1,000-200,000 reads of 36-250 bp; k=ceil(5% L)",
+            "requests": args.serve_requests, "slots": args.serve_slots, "runs": runs}
+
+
+def records_leg(job, db, reads, genome, res, ops):
+    """TSV records of every hit of the step just run: srm_job_records (device,
+    including the copy of the text to the host) against srm_format_records on the
+    host cores, byte for byte."""
+    import ctypes as C
+    import saruman_amd as sa
+    from saruman_amd import workload as wl
+    from saruman_amd._lib import ptr
+    lib = sa.load()
+    rid = np.ascontiguousarray(db.read_id, np.int32)
+    pos = np.ascontiguousarray(db.pos, np.int64)
+    text, hits = job.records(pos, read_id=rid, prefix="r")          # first call sizes buffers
+    buf = np.empty(len(text) + 1, np.uint8)
+    t0 = time.perf_counter()
+    text, hits = job.records(pos, read_id=rid, prefix="r", out=buf)
+    dev_ms = (time.perf_counter() - t0) * 1e3
+    fwd = np.frombuffer(reads.seq, np.uint8)
+    rev = wl.revcomp_all(reads)
+    roff = np.ascontiguousarray(reads.off[db.read_id], np.uint64)
+    g = np.frombuffer(genome.seq, np.uint8)
+    out = np.empty(len(text) + 1, np.uint8)
+    nh = C.c_longlong()
+    threads = 16
+    t0 = time.perf_counter()
+    need = lib.srm_format_records(int(db.n), ptr(res), ptr(ops), ptr(db.cand), ptr(g), ptr(pos),
+                                  ptr(fwd), ptr(rev), ptr(roff), ptr(rid), b"r", ptr(out),
+                                  out.size, C.byref(nh), threads)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    same = need == len(text) and out[:need].tobytes() == text and nh.value == hits
+    return {"device_ms": dev_ms, "host_ms": host_ms,
+            "host_threads": threads, "records": int(hits), "mb": len(text) / 1e6,
+            "identical": bool(same)}
 
 
 def device_filter_leg(ctx, index, genome, reads, chunk, host_strands):
@@ -187,8 +242,13 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tuning", action="store_true", help="ignore saruman_amd/tuning/*.json")
     ap.add_argument("--no-dfilter", action="store_true", help="skip the GPU q-gram filter leg")
+    ap.add_argument("--no-records", action="store_true", help="skip the record formatting leg")
+    ap.add_argument("--class-streams", type=int, default=None,
+                    help="run the band classes concurrently on N streams (overrides the tuning)")
     ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rates", default="4,16,64", help="Poisson arrivals per second (sweep)")
+    ap.add_argument("--serve-device-only", action="store_true",
+                    help="skip the host-filter serving run")
     ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -226,6 +286,10 @@ def main():
     index.close()
     from saruman_amd import tune
     tuning = None if args.no_tuning else tune.apply_cached(ctx)
+    if args.class_streams is not None:
+        tuning = dict(tuning or {"blocks_per_cu": [0] * 9, "merge_below": 0})
+        tuning["class_streams"] = args.class_streams
+        tune.apply(ctx, tuning)
     log(f"[bench] launch tuning: {tuning if tuning else 'defaults'}")
     t0 = time.perf_counter()
     ctx.set_genome(wl.genome_words(genome))
@@ -261,6 +325,14 @@ def main():
     n_mapped = len(mapped)
     log(f"[bench] banded: {ms_per_step:.3f} ms/step (device {np.median(dev_ms):.3f} ms, "
         f"band kernels {band_ms:.3f} ms), hits {int(hit.sum())}, reads mapped {n_mapped}")
+
+    # ---- records of every hit: formatted on the device vs on 16 host threads
+    records = None
+    if not args.no_records:
+        records = records_leg(job, db, reads, genome, res, ops)
+        log(f"[bench] records: device {records['device_ms']:.1f} ms (incl. the copy to the host), "
+            f"host {records['host_ms']:.1f} ms on {records['host_threads']} threads, "
+            f"{records['mb']:.0f} MB, identical: {records['identical']}")
 
     # ---- plain kernel on the same job
     plain = None
@@ -318,8 +390,7 @@ def main():
                    "reads": int(reads.n), "candidates": int(db.n), "reads_mapped": int(n_mapped),
                    "hits": int(hit.sum()), "match": 0, "mismatch": 1, "parallelism": "1 GPU",
                    "graph": not args.no_graph,
-                   "tuning": ({k: tuning[k] for k in ("blocks_per_cu", "merge_below")}
-                              if tuning else "defaults")},
+                   "tuning": tuning_key(tuning)},
         "gcups": ref_cells / (ms_per_step / 1e3) / 1e9,
         "band_gcups": bcells / (band_ms / 1e3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
@@ -337,6 +408,7 @@ def main():
                  "upload_s": t_upload, "download_s": t_d2h},
         "e2e_reads_per_s_pcie": n_mapped / (t_upload + ms_per_step / 1e3 + t_d2h),
         "gpu_filter": gpu_filter,
+        "records": records,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -212,8 +212,10 @@ int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src);
  * classes narrower than this run on the kernel of this class (fewer launches).
  */
 typedef struct srm_tuning {
-    int blocks_per_cu[9];
-    int merge_below;
+    int blocks_per_cu[9];   /* by band class (8 diagonals per unit); 0 = as many as fit */
+    int merge_below;        /* classes below this one run with it (0/1 = none) */
+    int class_streams;      /* streams the band classes run on side by side (up to 8);
+                               0 = default (4), 1 = one after another */
 } srm_tuning;
 int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *tuning);   /* NULL = defaults */
 /* after an eager srm_job_run: grid, tiles and event time (ms) of band class nw */
@@ -386,6 +388,8 @@ typedef struct srm_server_config {
     int format_threads;   /* host threads of record formatting (0 = all) */
     int chunk_value;      /* the reference's -s for every request */
     int kernel;           /* SRM_KERNEL_* */
+    int gpu_filter;       /* 1: q-gram filter on the device (srm_dfilter_*), 0: on the host */
+    int gpu_records;      /* 1: records formatted on the device (srm_job_records) */
 } srm_server_config;
 
 typedef struct srm_request_stats {

@@ -258,8 +258,15 @@ def format_records(batch: Batch, res: np.ndarray, ops: np.ndarray, lay=None) -> 
     return out
 
 
+def name_table(names, stride: int = 60) -> bytes:
+    """Fixed-stride NUL-terminated names (at most stride - 1 bytes each)."""
+    return b"".join(n.encode()[:stride - 1].ljust(stride, b"\0") for n in names)
+
+
 def align_batch(batch: Batch, kernel: str = "auto", device: int = 0, graph: bool = False,
-                repeats: int = 1) -> AlignResult:
+                repeats: int = 1, device_records: bool = False) -> AlignResult:
+    """Aligns a batch; records are formatted on the host.  device_records also
+    formats them on the device (srm_job_records) into extra["device_text"]."""
     lay = _layout(batch)
     rw = pack_codes(lay["rcat"])
     ww = pack_codes(lay["wcat"])
@@ -277,7 +284,12 @@ def align_batch(batch: Batch, kernel: str = "auto", device: int = 0, graph: bool
         bms, pms = job.kernel_ms()
         res, ops = job.download()
         paths = job.paths()
+        extra = {}
+        if device_records:
+            text, hits = job.records(batch.pos, read_id=np.arange(batch.n, dtype=np.int32),
+                                     names=name_table(batch.names), name_stride=60)
+            extra = {"device_text": text, "device_hits": hits}
     finally:
         job.close()
     recs = format_records(batch, res, ops, lay)
-    return AlignResult(res, ops, recs, ms, bms, pms, paths)
+    return AlignResult(res, ops, recs, ms, bms, pms, paths, extra)

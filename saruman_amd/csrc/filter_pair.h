@@ -182,6 +182,22 @@ struct Store {
     }
 };
 
+// does the ascending run flat[lo, min(hi, end)) -- cut at its first 0, as the
+// reference's 0-terminated scans are -- hold the position tg?
+SRM_HD inline bool run_has(const uint32_t *flat, uint64_t lo, uint64_t hi, uint64_t end, long tg)
+{
+    if (hi > end) hi = end;
+    if (lo >= hi || tg <= 0 || tg > 0xFFFFFFFFl || flat[lo] == 0) return false;
+    const uint32_t t = (uint32_t)tg;
+    uint64_t l = lo, h = hi;
+    while (l < h) {
+        const uint64_t m = (l + h) >> 1;
+        if (flat[m] < t) l = m + 1;
+        else h = m;
+    }
+    return l < hi && flat[l] == t;
+}
+
 // add_start_position, hash.c:40-178: 0 first entry, 1 rejected, 2 added or
 // flag lowered.  |pos - e| <= k is the reference's scan of pos + i, i in [-k, k].
 SRM_HD inline int add_start(Store &s, int32_t pos, int flag, int possible, int exact, int k)
@@ -243,69 +259,79 @@ SRM_HD inline void seashore(const FArgs &a, uint32_t r, Store &s)
                     if (s.ovf) return;
                 }
                 int possible = 1;
-                // first pass, hash.c:606-692
+                // first pass, hash.c:606-692.  For every err the reference scans
+                // D(b, off), then D(b, 1), D(b, 2), ... up to the next 0 (Q12): the
+                // element at a's index, then one ascending run of list b (or of the
+                // list after b when b is empty).  A run holds each position once, so
+                // a binary search per err finds what the scan finds, in scan order.
                 for (int b = aa + 1; b < half; ++b) {
+                    const uint32_t P0 = D(b, off);
+                    if (P0 == 0) continue;                 // the scan never starts
                     const long rstart = (long)b * q + 1;
+                    uint64_t lo = (uint64_t)a.list_start[lb + b] + 1, hi = lo;
+                    const uint64_t nb = a.list_start[lb + b + 1];
+                    if (lo < nb) {
+                        hi = nb - 1;                       // list b from index 1
+                    } else if ((uint64_t)lb + b + 2 <= a.list_base[a.n_reads]) {
+                        hi = (uint64_t)a.list_start[lb + b + 2] - 1;   // the next list
+                    }
+                    int flag;
+                    long pos;
+                    if (b == half - 1 && fqm) {
+                        flag = 1 + dr;
+                        pos = gs + 1;
+                    } else if (aa == half - 1) {
+                        flag = ((long)Pa + q > L ? 3 : 2) + dr;
+                        pos = gs - k + 1;
+                    } else if (fqm) {
+                        flag = 1 + dr;
+                        pos = gs + 1;
+                    } else {
+                        flag = 3 + dr;
+                        pos = gs - k + 1;
+                    }
                     for (int err = -k; err <= k; ++err) {
-                        uint32_t Pb = D(b, off);
-                        uint64_t nxt = 0;
-                        while (Pb > 0) {
-                            if ((long)Pb == gs + rstart + err) {
-                                int flag;
-                                long pos;
-                                if (b == half - 1 && fqm) {
-                                    flag = 1 + dr;
-                                    pos = gs + 1;
-                                } else if (aa == half - 1) {
-                                    flag = ((long)Pa + q > L ? 3 : 2) + dr;
-                                    pos = gs - k + 1;
-                                } else if (fqm) {
-                                    flag = 1 + dr;
-                                    pos = gs + 1;
-                                } else {
-                                    flag = 3 + dr;
-                                    pos = gs - k + 1;
-                                }
-                                ++possible;
-                                add_start(s, (int32_t)pos, flag, possible, P.exact, k);
-                                if (s.ovf) return;
-                            }
-                            ++nxt;
-                            Pb = D(b, nxt);
+                        const long tg = gs + rstart + err;
+                        if ((long)P0 == tg) {
+                            ++possible;
+                            add_start(s, (int32_t)pos, flag, possible, P.exact, k);
+                            if (s.ovf) return;
+                        }
+                        if (run_has(a.flat, lo, hi, flat_end, tg)) {
+                            ++possible;
+                            add_start(s, (int32_t)pos, flag, possible, P.exact, k);
+                            if (s.ovf) return;
                         }
                     }
                 }
-                // second pass, hash.c:695-794
+                // second pass, hash.c:695-794: each err scans list half + j from its
+                // start up to its 0
                 for (int j = 0; j < half; ++j) {
                     const int lim = half - 1 - j - aa < 0 ? -(half - 1 - j - aa) : half - 1 - j - aa;
                     const long nin = (long)(half - j - 1) * q + 1 + rest;
+                    const uint64_t lo = a.list_start[lb + half + j];
+                    const uint64_t hi = (uint64_t)a.list_start[lb + half + j + 1] - 1;
+                    int flag;
+                    long pos;
+                    if (j == 0 && fqm) {
+                        flag = 1 + dr;
+                        pos = gs + 1;
+                    } else if (j == 0) {
+                        flag = (nin + q > L ? 3 : 2) + dr;
+                        pos = gs - k + 1;
+                    } else if (fqm) {
+                        flag = 1 + dr;
+                        pos = gs + 1;
+                    } else {
+                        flag = 3 + dr;
+                        pos = gs - k + 1;
+                    }
                     for (int err = -lim; err <= lim; ++err) {
                         const int aerr = (j - aa < 0) ? -err : err;
-                        uint64_t so = 0;
-                        uint32_t Pb = D(half + j, so);
-                        while (Pb > 0) {
-                            if ((long)Pb == gs + nin + aerr) {
-                                int flag;
-                                long pos;
-                                if (j == 0 && fqm) {
-                                    flag = 1 + dr;
-                                    pos = gs + 1;
-                                } else if (j == 0) {
-                                    flag = (nin + q > L ? 3 : 2) + dr;
-                                    pos = gs - k + 1;
-                                } else if (fqm) {
-                                    flag = 1 + dr;
-                                    pos = gs + 1;
-                                } else {
-                                    flag = 3 + dr;
-                                    pos = gs - k + 1;
-                                }
-                                ++possible;
-                                add_start(s, (int32_t)pos, flag, possible, P.exact, k);
-                                if (s.ovf) return;
-                            }
-                            ++so;
-                            Pb = D(half + j, so);
+                        if (run_has(a.flat, lo, hi, flat_end, gs + nin + aerr)) {
+                            ++possible;
+                            add_start(s, (int32_t)pos, flag, possible, P.exact, k);
+                            if (s.ovf) return;
                         }
                     }
                 }

@@ -172,8 +172,8 @@ __global__ __launch_bounds__(256) void plain_kernel(PlainArgs a)
 // word (q, i) of lane l sits at ((q * BMAX + i) * 64 + l) and holds band cell
 // (diagonal) i of columns 8q .. 8q+7 (column 8q + j in bits 4j..4j+3).  The
 // forward pass fills the BMAX words of a block every 8 columns with coalesced
-// 256-byte wave stores at immediate offsets; a traceback step along a diagonal
-// stays inside the words it already has.
+// 256-byte wave stores at immediate offsets; the traceback keeps the words of
+// five diagonals in three blocks around its walk in registers.
 template <int BMAX>
 __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, int i, int q,
                                               int lane)
@@ -181,26 +181,11 @@ __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, 
     return scr[((size_t)q * BMAX + i) * 64 + lane];
 }
 
-// 16 codes of a sequence starting at a multiple of 16 at or below idx
-struct CodeCache {
-    int64_t base = INT64_MIN;
-    uint32_t word = 0;
-    __device__ __forceinline__ int at(const uint32_t *__restrict__ words, int64_t org, int idx)
-    {
-        const int64_t b = (int64_t)(idx & ~15);
-        if (b != base) {
-            base = b;
-            word = fetch16(words, org + b);
-        }
-        return (int)((word >> ((idx & 15) * 2)) & 3u);
-    }
-};
-
 // occupancy each band class is compiled for (waves per SIMD): the register
 // budget is 512 / waves VGPRs
 template <int NW>
 struct BandWaves {
-    static constexpr int value = NW <= 1 ? 8 : NW <= 2 ? 5 : NW <= 3 ? 4 : NW <= 5 ? 3 : 2;
+    static constexpr int value = NW <= 1 ? 5 : NW <= 3 ? 4 : NW <= 5 ? 3 : 2;
 };
 
 template <int NW, bool UNIT>
@@ -357,87 +342,173 @@ void band_kernel(BandArgs a)
         }
         if (active) column(L, true);
 
-        // ---- score and traceback (cuda-functions.cu:495-534, App. A.3)
+        // ---- score (cuda-functions.cu:495-497)
+        int score = K;
         if (active) {
-            srm_result r;
             const int iend = w - L - dlo;
-            int score = K;
             if (iend >= 0 && iend < BMAX) {
                 const uint32_t wd0 = band_word<BMAX>(scr, iend, L >> 3, lane);
                 score = imin((int)((wd0 >> ((L & 7) * 4)) & 15u), K);
             }
-            r.score = (int16_t)score;
-            r.n_ops = 0;
-            r.end_row = (uint16_t)w;
-            r.end_col = (uint16_t)L;
-            if (score <= k) {
-                const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
-                uint32_t *__restrict__ ops = a.ops + cd.ops_off;
-                const int cap = w + k;
-                int row = w, col = L, n = 0;
-                uint32_t opw = 0;
-                // cached words of diagonals ci-1, ci, ci+1 in column block cq (Wm, W0,
-                // Wp) and of ci, ci+1 in block cq-1 (P0, Pp)
-                int ci = -1, cq = -1;
-                uint32_t Wm = 0, W0 = 0, Wp = 0, P0 = 0, Pp = 0;
-                CodeCache rcc, gcc;
-                for (;;) {
-                    if (stop_row0 && row == 0) break;
-                    const int i = row - col - dlo;
-                    const int q = col >> 3;
-                    if (i != ci || q != cq) {
-                        W0 = band_word<BMAX>(scr, i, q, lane);
-                        Wm = i >= 1 ? band_word<BMAX>(scr, i - 1, q, lane) : 0xFFFFFFFFu;
-                        Wp = i + 1 < BMAX ? band_word<BMAX>(scr, i + 1, q, lane) : 0xFFFFFFFFu;
-                        if (q >= 1) {
-                            P0 = band_word<BMAX>(scr, i, q - 1, lane);
-                            Pp = i + 1 < BMAX ? band_word<BMAX>(scr, i + 1, q - 1, lane) : 0xFFFFFFFFu;
-                        }
-                        ci = i;
-                        cq = q;
-                    }
-                    const uint32_t sh = (uint32_t)(col & 7) * 4u;
-                    // a nibble of 15 outside the band reads as K (never equal to v <= k)
-                    const int vv = (int)((W0 >> sh) & 15u);
-                    int Dg, Lf, Tp;
-                    if (col != 0) {
-                        uint32_t dgw, lfw;
-                        uint32_t psh;
-                        if (col & 7) {
-                            dgw = W0;
-                            lfw = Wp;
-                            psh = sh - 4u;
-                        } else {
-                            dgw = P0;
-                            lfw = Pp;
-                            psh = 28u;
-                        }
-                        Dg = row >= 1 ? imin((int)((dgw >> psh) & 15u), K) : SRM_FOREIGN;
-                        Lf = imin((int)((lfw >> psh) & 15u), K);
-                    } else {
-                        Dg = SRM_COL0;
-                        Lf = SRM_COL0;
-                    }
-                    Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
-                    const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
-                    const bool eq = gcc.at(a.win_words, (int64_t)cd.win_off, ht) ==
-                                    rcc.at(a.read_words, rbase, wd);
-                    const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
-                    if (op == 3 || n >= cap) break;
-                    opw |= (uint32_t)op << ((n & 15) * 2);
-                    ++n;
-                    if ((n & 15) == 0) {
-                        ops[(n >> 4) - 1] = opw;
-                        opw = 0;
-                    }
-                    row -= (op != 2);
-                    col -= (op != 1);
-                }
-                if (n & 15) ops[n >> 4] = opw;
-                r.n_ops = (uint16_t)n;
-                r.end_row = (uint16_t)row;
-                r.end_col = (uint16_t)col;
+        }
+
+        // ---- traceback (cuda-functions.cu:504-534, App. A.3) in phases of 8 steps.
+        // A walk moves at most one column and one row per step, so at the start of a
+        // phase every lane loads what its next 8 steps can touch, one phase ahead:
+        //  - band words of diagonals bi-2..bi+2 in blocks bq, bq-1, bq-2 (BA, BB, BC);
+        //    entering block bq-1 rotates them and fetches block bq-3;
+        //  - 32 read codes and 32 window codes covering the next phase.
+        // The tiles hold candidates of equal L mod 8, so diagonal walks cross blocks
+        // together and the rotation loads of all lanes overlap the next phase; an
+        // indel that leaves the cached diagonals reloads at once (rare).
+        bool walking = active && score <= k;
+        const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
+        uint32_t *__restrict__ ops = a.ops + (walking ? cd.ops_off : 0u);
+        const int cap = w + k;
+        const int64_t gorg = (int64_t)cd.win_off;
+        int row = w, col = L, n = 0;
+        uint32_t opw = 0;
+        uint32_t BA[5], BB[5], BC[5];
+        int bq = -8, bi = 0, vlo = 1, vhi = 0;     // nothing cached yet
+        auto load_block = [&](uint32_t (&X)[5], int q, int i) {
+#pragma unroll
+            for (int d = 0; d < 5; ++d) {
+                const int ii = i - 2 + d;
+                X[d] = (q >= 0 && ii >= 0 && ii < BMAX) ? band_word<BMAX>(scr, ii, q, lane)
+                                                         : 0xFFFFFFFFu;
             }
+        };
+        auto load_all = [&](int q, int i) {
+            load_block(BA, q, i);
+            load_block(BB, q - 1, i);
+            load_block(BC, q - 2, i);
+            bq = q;
+            bi = i;
+            vlo = i - 2;
+            vhi = i + 2;
+        };
+        // code windows: 2 words from absolute code index 16 * base.  A phase that
+        // starts at column c reads codes c-9 .. c-1; its window is fetched at the
+        // previous phase's start c' <= c + 8 from c' - 17, so 32 codes cover it.
+        uint32_t RC[2] = {0u, 0u}, GC[2] = {0u, 0u}, RN[2] = {0u, 0u}, GN[2] = {0u, 0u};
+        // (word indices: code arrays stay below 2^35 codes)
+        int rcb = 0, gcb = 0, rnb = 0, gnb = 0;
+        auto load_codes = [&](uint32_t (&X)[2], int &xb, const uint32_t *words, int64_t abs_lo) {
+            xb = (int)(abs_lo >> 4);
+            X[0] = words[xb];
+            X[1] = words[xb + 1];
+        };
+        auto code2 = [&](const uint32_t (&X)[2], int xb, int64_t abs_idx) -> int {
+            const int rel = (int)(abs_idx - 16 * (int64_t)xb);   // 0..31
+            const uint32_t wv = rel < 16 ? X[0] : X[1];
+            return (int)((wv >> ((rel & 15) * 2)) & 3u);
+        };
+        if (walking) {
+            load_all(L >> 3, w - L - dlo);
+            load_codes(RC, rcb, a.read_words, rbase + imax(L - 9, 0));
+            load_codes(GC, gcb, a.win_words, gorg + imax(w - 9, 0));
+            load_codes(RN, rnb, a.read_words, rbase + imax(L - 17, 0));
+            load_codes(GN, gnb, a.win_words, gorg + imax(w - 17, 0));
+        }
+        bool first = true;
+        while (__any(walking)) {
+            if (walking && !first) {
+                // phase start: this phase's codes were fetched one phase ago
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    RC[j] = RN[j];
+                    GC[j] = GN[j];
+                }
+                rcb = rnb;
+                gcb = gnb;
+                load_codes(RN, rnb, a.read_words, rbase + imax(col - 17, 0));
+                load_codes(GN, gnb, a.win_words, gorg + imax(row - 17, 0));
+                const int q = col >> 3, i = row - col - dlo;
+                if (q == bq - 1 && i == bi && vlo == i - 2 && vhi == i + 2) {
+#pragma unroll
+                    for (int d = 0; d < 5; ++d) {
+                        BA[d] = BB[d];
+                        BB[d] = BC[d];
+                    }
+                    load_block(BC, q - 2, i);
+                    bq = q;
+                } else if (!(q == bq && i == bi && vlo == i - 2 && vhi == i + 2)) {
+                    load_all(q, i);
+                }
+            }
+            first = false;
+            for (int stp = 0; stp < 8; ++stp) {
+                if (!walking) break;
+                if (stop_row0 && row == 0) {
+                    walking = false;
+                    break;
+                }
+                const int i = row - col - dlo;
+                // an indel moves the walk by one diagonal: shift the cached words
+                if (i == bi - 1) {
+#pragma unroll
+                    for (int d = 4; d > 0; --d) {
+                        BA[d] = BA[d - 1];
+                        BB[d] = BB[d - 1];
+                        BC[d] = BC[d - 1];
+                    }
+                    bi = i;
+                    vhi = imin(vhi, i + 2);
+                } else if (i == bi + 1) {
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        BA[d] = BA[d + 1];
+                        BB[d] = BB[d + 1];
+                        BC[d] = BC[d + 1];
+                    }
+                    bi = i;
+                    vlo = imax(vlo, i - 2);
+                }
+                const int q = col >> 3;
+                const int qd = col > 0 ? (col - 1) >> 3 : q;   // block of column col - 1
+                if (i != bi || i - 1 < vlo || i + 1 > vhi || bq - qd > 2 || q > bq) load_all(q, i);
+                const int o = bq - q, od = bq - qd;           // 0, 1 or 2: BA, BB, BC
+                const uint32_t Wm = o == 0 ? BA[1] : o == 1 ? BB[1] : BC[1];
+                const uint32_t W0 = o == 0 ? BA[2] : o == 1 ? BB[2] : BC[2];
+                const uint32_t D0 = od == 0 ? BA[2] : od == 1 ? BB[2] : BC[2];
+                const uint32_t Dp = od == 0 ? BA[3] : od == 1 ? BB[3] : BC[3];
+                const uint32_t sh = (uint32_t)(col & 7) * 4u;
+                const uint32_t psh = (uint32_t)((col - 1) & 7) * 4u;
+                // a nibble of 15 outside the band reads as K (never equal to v <= k)
+                const int vv = (int)((W0 >> sh) & 15u);
+                int Dg, Lf;
+                if (col != 0) {
+                    Dg = row >= 1 ? imin((int)((D0 >> psh) & 15u), K) : SRM_FOREIGN;
+                    Lf = imin((int)((Dp >> psh) & 15u), K);
+                } else {
+                    Dg = SRM_COL0;
+                    Lf = SRM_COL0;
+                }
+                const int Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
+                const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
+                const bool eq = code2(GC, gcb, gorg + ht) == code2(RC, rcb, rbase + wd);
+                const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
+                if (op == 3 || n >= cap) {
+                    walking = false;
+                    break;
+                }
+                opw |= (uint32_t)op << ((n & 15) * 2);
+                ++n;
+                if ((n & 15) == 0) {
+                    ops[(n >> 4) - 1] = opw;
+                    opw = 0;
+                }
+                row -= (op != 2);
+                col -= (op != 1);
+            }
+        }
+        if (active) {
+            srm_result r;
+            r.score = (int16_t)score;
+            r.n_ops = (uint16_t)n;
+            r.end_row = (uint16_t)row;
+            r.end_col = (uint16_t)col;
+            if (score <= k && (n & 15)) ops[n >> 4] = opw;
             a.res[t] = r;
         }
     }

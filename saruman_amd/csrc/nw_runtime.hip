@@ -128,6 +128,7 @@ struct BandClass {
     int nw = 0;
     uint32_t tile_begin = 0, ntiles = 0;
     int maxL = 0;
+    size_t scratch_off = 0;   // words into the job's band scratch
     int blocks = 0;
 };
 struct PlainChunk {
@@ -158,7 +159,11 @@ struct srm_job {
     size_t mat_budget_elems = 0;
     // timing
     hipEvent_t ev[3] = {};
-    hipEvent_t cev[9] = {};          // around each band-class launch
+    hipEvent_t cs[9] = {}, ce[9] = {};   // around each band-class launch
+    hipEvent_t fork_ev = nullptr;        // concurrent classes: fork and joins
+    hipEvent_t join_ev[8] = {};
+    std::vector<hipStream_t> side;       // streams of concurrent band classes
+    int class_streams = 0;               // at the last upload
     float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f;
     float class_ms[9] = {};          // by NW (1..8), last eager run
     // graph
@@ -243,8 +248,22 @@ int run_launches(srm_job *j, int kernel, bool timed)
     if (timed) HIP_TRY(hipEventRecord(j->ev[0], st));
     if (use_band && !j->classes.empty()) {
         HIP_TRY(srm::launch_reset((uint32_t *)j->d_queue.p, (int)j->classes.size(), st));
-        for (size_t c = 0; c < j->classes.size(); ++c) {
+        // with class_streams > 1 the classes run side by side on their own streams
+        // (each with its own scratch), widest band first; otherwise in order on st
+        const int S = std::min<int>(j->class_streams, (int)j->classes.size());
+        std::vector<size_t> order(j->classes.size());
+        for (size_t c = 0; c < order.size(); ++c) order[c] = c;
+        if (S > 1) {
+            std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+                return j->classes[x].nw > j->classes[y].nw;
+            });
+            HIP_TRY(hipEventRecord(j->fork_ev, st));
+            for (int q = 0; q < S; ++q) HIP_TRY(hipStreamWaitEvent(j->side[(size_t)q], j->fork_ev, 0));
+        }
+        for (size_t o = 0; o < order.size(); ++o) {
+            const size_t c = order[o];
             const BandClass &bc = j->classes[c];
+            hipStream_t cst = S > 1 ? j->side[o % (size_t)S] : st;
             srm::BandArgs a;
             a.ma = j->ma;
             a.mm = j->mm;
@@ -257,14 +276,19 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.win_words = j->resident_genome
                               ? ctx->genome_ptr()
                               : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
-            a.scratch = (uint32_t *)j->d_scratch.p;
+            a.scratch = (uint32_t *)j->d_scratch.p + bc.scratch_off;
             a.scratch_words_per_wave = band_scratch_words(bc);
             a.res = (srm_result *)j->d_res.p;
             a.ops = (uint32_t *)j->d_ops.p;
-            if (timed && c == 0) HIP_TRY(hipEventRecord(j->cev[0], st));
-            HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, st));
-            if (timed) HIP_TRY(hipEventRecord(j->cev[c + 1], st));
+            if (timed) HIP_TRY(hipEventRecord(j->cs[c], cst));
+            HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, cst));
+            if (timed) HIP_TRY(hipEventRecord(j->ce[c], cst));
         }
+        if (S > 1)
+            for (int q = 0; q < S; ++q) {
+                HIP_TRY(hipEventRecord(j->join_ev[q], j->side[(size_t)q]));
+                HIP_TRY(hipStreamWaitEvent(st, j->join_ev[q], 0));
+            }
     }
     if (timed) HIP_TRY(hipEventRecord(j->ev[1], st));
     auto plain = [&](const std::vector<PlainChunk> &chunks, DevBuf &idx, DevBuf &off,
@@ -404,6 +428,7 @@ int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *t)
     for (int i = 0; i < 9; ++i)
         if (t->blocks_per_cu[i] < 0 || t->blocks_per_cu[i] > 64) return fail(SRM_EINVAL, "blocks_per_cu");
     if (t->merge_below < 0 || t->merge_below > 8) return fail(SRM_EINVAL, "merge_below");
+    if (t->class_streams < 0 || t->class_streams > 8) return fail(SRM_EINVAL, "class_streams");
     ctx->tuning = *t;
     return SRM_OK;
 }
@@ -439,14 +464,15 @@ int srm_job_create(srm_ctx *ctx, srm_job **out)
     HIP_TRY(hipSetDevice(ctx->device));
     srm_job *j = new srm_job();
     j->ctx = ctx;
-    for (auto *arr : {&j->ev[0], &j->cev[0]}) {
-        const int cnt = arr == &j->ev[0] ? 3 : 9;
-        for (int i = 0; i < cnt; ++i) {
-            if (hipEventCreate(&arr[i]) != hipSuccess) {
-                delete j;
-                return fail(SRM_EDEVICE, "hipEventCreate failed");
-            }
-        }
+    bool ok = true;
+    for (auto &e : j->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    for (auto &e : j->cs) ok = ok && hipEventCreate(&e) == hipSuccess;
+    for (auto &e : j->ce) ok = ok && hipEventCreate(&e) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&j->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (auto &e : j->join_ev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        srm_job_destroy(j);
+        return fail(SRM_EDEVICE, "hipEventCreate failed");
     }
     *out = j;
     return SRM_OK;
@@ -467,10 +493,16 @@ void srm_job_destroy(srm_job *j)
                       &j->d_rec_out, &j->d_rec_pos, &j->d_rec_id, &j->d_rec_names,
                       &j->d_rec_hits};
     for (DevBuf *b : bufs) c->release(*b);
-    for (auto &e : j->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto &e : j->cev)
-        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t q : j->side) {
+        (void)hipStreamSynchronize(q);
+        (void)hipStreamDestroy(q);
+    }
+    for (auto *arr : {&j->ev[0], &j->cs[0], &j->ce[0], &j->join_ev[0]}) {
+        const int cnt = arr == &j->ev[0] ? 3 : arr == &j->join_ev[0] ? 8 : 9;
+        for (int i = 0; i < cnt; ++i)
+            if (arr[i]) (void)hipEventDestroy(arr[i]);
+    }
+    if (j->fork_ev) (void)hipEventDestroy(j->fork_ev);
     delete j;
 }
 
@@ -538,34 +570,58 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     j->n_bytes = bytes_idx.size();
     j->ops_words = ops_end;
 
-    // ---- banded plan: sort by (class, L desc), 64-candidate tiles
+    // ---- banded plan: sort by (class, L mod 8, L desc), 64-candidate tiles that
+    // never mix L mod 8: a tile's lanes then cross the 8-column blocks of their
+    // band scratch at the same traceback steps (a walk moves one column per
+    // diagonal step), so the wave reloads band words together
     std::vector<int> cls(n, 0);
     const int merge_below = std::max(1, std::min(8, ctx->tuning.merge_below));
     for (uint32_t t : band_idx) cls[t] = std::max((band_cells(cand[t]) + 7) / 8, merge_below);
     std::stable_sort(band_idx.begin(), band_idx.end(), [&](uint32_t a, uint32_t b) {
         if (cls[a] != cls[b]) return cls[a] < cls[b];
+        if ((cand[a].L & 7) != (cand[b].L & 7)) return (cand[a].L & 7) < (cand[b].L & 7);
         return cand[a].L > cand[b].L;
     });
     std::vector<uint32_t> tiles;   // uint4 per tile
     j->classes.clear();
     size_t scratch_words = 0;
+    // band classes run side by side on 4 streams unless the tuning says otherwise
+    // (1 = one after another on the context's stream)
+    const int want_streams = ctx->tuning.class_streams == 0 ? 4 : ctx->tuning.class_streams;
+    const bool concurrent = want_streams > 1;
+    j->class_streams = concurrent ? std::min(want_streams, 8) : 0;
+    while ((int)j->side.size() < j->class_streams) {   // default priority, like every stream here
+        hipStream_t q = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        j->side.push_back(q);
+    }
     {
         size_t i = 0;
         while (i < band_idx.size()) {
             BandClass bc;
             bc.nw = cls[band_idx[i]];
             bc.tile_begin = (uint32_t)(tiles.size() / 4);
-            bc.maxL = cand[band_idx[i]].L;
+            bc.maxL = 0;
             size_t e = i;
             while (e < band_idx.size() && cls[band_idx[e]] == bc.nw) ++e;
-            for (size_t s = i; s < e; s += 64) {
-                uint32_t cnt = (uint32_t)std::min<size_t>(64, e - s);
-                uint32_t kmax = 0;
-                for (size_t u = s; u < s + cnt; ++u) kmax = std::max<uint32_t>(kmax, cand[band_idx[u]].k);
-                tiles.push_back((uint32_t)s);
-                tiles.push_back(cnt);
-                tiles.push_back(cand[band_idx[s]].L);   // sorted by L: the tile's max
-                tiles.push_back(kmax);
+            for (size_t g = i; g < e;) {
+                // one run of equal L mod 8
+                size_t ge = g;
+                while (ge < e && (cand[band_idx[ge]].L & 7) == (cand[band_idx[g]].L & 7)) ++ge;
+                for (size_t s = g; s < ge; s += 64) {
+                    const uint32_t cnt = (uint32_t)std::min<size_t>(64, ge - s);
+                    uint32_t kmax = 0, lmax = 0;
+                    for (size_t u = s; u < s + cnt; ++u) {
+                        kmax = std::max<uint32_t>(kmax, cand[band_idx[u]].k);
+                        lmax = std::max<uint32_t>(lmax, cand[band_idx[u]].L);
+                    }
+                    tiles.push_back((uint32_t)s);
+                    tiles.push_back(cnt);
+                    tiles.push_back(lmax);
+                    tiles.push_back(kmax);
+                    bc.maxL = std::max(bc.maxL, (int)lmax);
+                }
+                g = ge;
             }
             bc.ntiles = (uint32_t)(tiles.size() / 4) - bc.tile_begin;
             // persistent grid: as many 4-wave blocks as stay resident on every CU
@@ -575,7 +631,14 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             if (tuned > 0) per_cu = std::min(per_cu, tuned);
             int max_blocks = std::max(1, ctx->num_cus * per_cu);
             bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
-            scratch_words = std::max(scratch_words, (size_t)bc.blocks * 4 * band_scratch_words(bc));
+            const size_t sw = (size_t)bc.blocks * 4 * band_scratch_words(bc);
+            if (concurrent) {          // classes side by side: a scratch region each
+                bc.scratch_off = scratch_words;
+                scratch_words += sw;
+            } else {
+                bc.scratch_off = 0;
+                scratch_words = std::max(scratch_words, sw);
+            }
             j->classes.push_back(bc);
             i = e;
         }
@@ -662,7 +725,7 @@ int srm_job_run(srm_job *j, int kernel)
     for (auto &x : j->class_ms) x = 0.f;
     if (kernel != SRM_KERNEL_PLAIN)   // the class events are recorded only with band launches
         for (size_t c = 0; c < j->classes.size(); ++c)
-            HIP_TRY(hipEventElapsedTime(&j->class_ms[j->classes[c].nw], j->cev[c], j->cev[c + 1]));
+            HIP_TRY(hipEventElapsedTime(&j->class_ms[j->classes[c].nw], j->cs[c], j->ce[c]));
     return SRM_OK;
 }
 

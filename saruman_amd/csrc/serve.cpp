@@ -3,6 +3,7 @@
 // A server keeps the packed genome resident on one GPU and runs requests (read
 // batches of any size) through three overlapping stages:
 //   filter   (host thread)   q-gram filter of both strands -> candidates, codes
+//                            (on the host, or on the device with its own stream)
 //   admit    (host thread)   a request enters as soon as one of n_slots slots is
 //                            free and its device footprint fits the HBM budget
 //   slot k   (host thread)   upload, persistent banded kernel, download and record
@@ -71,6 +72,8 @@ struct srm_server {
     srm_server_config cfg{};
     std::string genome;               // ASCII, for records
     srm_index *index = nullptr;
+    srm_ctx *fctx = nullptr;          // the device filter's context (own stream)
+    srm_dfilter *dfilter = nullptr;   // set when the filter runs on the device
     std::vector<srm_ctx *> ctx;       // one per slot; slot 0 owns the resident genome
     std::vector<srm_job *> job;
     Clock::time_point t0;
@@ -109,9 +112,12 @@ void srm_server::prepare(Request &r)
     // both strands through the reference filter semantics
     for (int s = 0; s < 2; ++s) {
         srm_filter *F = nullptr;
-        int rc = srm_filter_run(index, genome.data(), genome.size(), nullptr, r.reads.data(),
-                                r.off.data(), r.len.data(), r.k.data(), n, s ? 4 : 0,
-                                cfg.chunk_value, cfg.filter_threads, &F);
+        int rc = dfilter ? srm_dfilter_run(dfilter, nullptr, r.reads.data(), r.off.data(),
+                                           r.len.data(), r.k.data(), n, s ? 4 : 0,
+                                           cfg.chunk_value, &F)
+                         : srm_filter_run(index, genome.data(), genome.size(), nullptr,
+                                          r.reads.data(), r.off.data(), r.len.data(), r.k.data(),
+                                          n, s ? 4 : 0, cfg.chunk_value, cfg.filter_threads, &F);
         if (rc != SRM_OK) {
             r.st.status = rc;
             return;
@@ -157,6 +163,8 @@ void srm_server::prepare(Request &r)
     const size_t waves = std::min<size_t>((nc + 63) / 64, 256 * 16);
     r.dev_bytes = r.read_words.size() * 4 + nc * (sizeof(srm_cand) + sizeof(srm_result) + 4 * 4) +
                   r.ops_total * 4 + waves * (maxL + 1) * 8 * 64 * 4 + (1u << 20);
+    if (cfg.gpu_records)   // positions, ids, lengths, offsets, entries and the text
+        r.dev_bytes += nc * (8 + 4 + 8 + 8 + 16 + 2 * maxL + 48);
     r.st.n_reads = (long long)n;
     r.st.n_candidates = (long long)nc;
     r.st.dev_bytes = (long long)r.dev_bytes;
@@ -174,7 +182,9 @@ void srm_server::run_on_slot(int s, Request &r)
                             nullptr, 0, nullptr, 0, nullptr, 0);
         if (rc == SRM_OK) rc = srm_job_run(j, cfg.kernel);
         if (rc == SRM_OK) r.st.device_ms = srm_job_last_ms(j);
-        if (rc == SRM_OK) rc = srm_job_download(j, res.data(), ops.data(), ops.size());
+        if (rc == SRM_OK)   // the transcripts stay on the device when it formats the records
+            rc = srm_job_download(j, res.data(), cfg.gpu_records ? nullptr : ops.data(),
+                                  cfg.gpu_records ? 0 : ops.size());
     }
     r.st.t_aligned = now();
     if (rc != SRM_OK) {
@@ -199,7 +209,28 @@ void srm_server::run_on_slot(int s, Request &r)
         }
     }
     r.st.n_perfect = (long long)(r.perf_read[0].size() + r.perf_read[1].size());
-    if (n > 0) {
+    if (n > 0 && cfg.gpu_records) {
+        long long hits = 0;
+        const size_t base = out.size();
+        size_t cap = (size_t)n * 64;
+        for (int pass = 0; pass < 2; ++pass) {
+            out.resize(base + cap);
+            long long got = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r",
+                                            &out[base], cap, &hits);
+            if (got < 0) {
+                r.st.status = (int)got;
+                return;
+            }
+            if ((size_t)got <= cap) {
+                out.resize(base + (size_t)got);
+                break;
+            }
+            cap = (size_t)got;   // the first call was a size query
+        }
+        r.st.n_hits = hits;
+        for (int t = 0; t < n; ++t)
+            if (res[(size_t)t].score <= (int)r.cand[(size_t)t].k) mapped[(size_t)r.rid[(size_t)t]] = 1;
+    } else if (n > 0) {
         long long hits = 0;
         long long need = srm_format_records(n, res.data(), ops.data(), r.cand.data(), genome.data(),
                                             r.pos.data(), r.reads.data(), r.reads_rev.data(),
@@ -327,6 +358,16 @@ int srm_server_create(const srm_server_config *cfg, const char *genome, size_t g
         delete S;
         return rc;
     }
+    if (cfg->gpu_filter) {
+        rc = srm_ctx_create(cfg->device, 0, &S->fctx);
+        if (rc == SRM_OK) rc = srm_dfilter_create(S->fctx, S->index, genome, glen, &S->dfilter);
+        if (rc != SRM_OK) {
+            if (S->fctx) srm_ctx_destroy(S->fctx);
+            srm_index_destroy(S->index);
+            delete S;
+            return rc;
+        }
+    }
     std::vector<uint32_t> gw((glen + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
     srm_pack_codes(genome, glen, gw.data(), SRM_CODE_PAD);
     for (int s = 0; s < cfg->n_slots; ++s) {
@@ -342,6 +383,8 @@ int srm_server_create(const srm_server_config *cfg, const char *genome, size_t g
                 srm_job_destroy(S->job[i]);
                 srm_ctx_destroy(S->ctx[i]);
             }
+            srm_dfilter_destroy(S->dfilter);
+            if (S->fctx) srm_ctx_destroy(S->fctx);
             srm_index_destroy(S->index);
             delete S;
             return rc;
@@ -458,6 +501,8 @@ void srm_server_destroy(srm_server *S)
         srm_job_destroy(S->job[i]);
         srm_ctx_destroy(S->ctx[i]);
     }
+    srm_dfilter_destroy(S->dfilter);
+    if (S->fctx) srm_ctx_destroy(S->fctx);
     srm_index_destroy(S->index);
     delete S;
 }

@@ -19,7 +19,8 @@ from .workload import Genome, Reads, filter_params, k_for
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("n_slots", C.c_int), ("hbm_budget", C.c_size_t),
                 ("filter_threads", C.c_int), ("format_threads", C.c_int),
-                ("chunk_value", C.c_int), ("kernel", C.c_int)]
+                ("chunk_value", C.c_int), ("kernel", C.c_int), ("gpu_filter", C.c_int),
+                ("gpu_records", C.c_int)]
 
 
 class RequestStats(C.Structure):
@@ -42,10 +43,12 @@ def qs_for_lengths(Lmin: int, Lmax: int, k=None, k_rate=0.05):
 
 class Server:
     def __init__(self, genome: Genome, qs, n_slots=4, hbm_budget=0, filter_threads=0,
-                 format_threads=0, chunk_value=200_000, kernel="auto", device=0):
+                 format_threads=0, chunk_value=200_000, kernel="auto", device=0,
+                 gpu_filter=False, gpu_records=False):
         self.lib = _lib.load()
         cfg = ServerConfig(device, n_slots, hbm_budget, filter_threads, format_threads,
-                           chunk_value, _lib.SRM_KERNEL[kernel])
+                           chunk_value, _lib.SRM_KERNEL[kernel], int(gpu_filter),
+                           int(gpu_records))
         q = np.array(sorted(set(qs)), np.int32)
         self._g = np.frombuffer(genome.seq, np.uint8)
         h = C.c_void_p()

@@ -121,3 +121,65 @@ def test_filter_leak_after_skip():
     got = wl.run_filter(wl.Index(genome, [wl.filter_params(L, k)[0]]), genome, reads, 0, 100)
     i = got.read.tolist().index(2)
     assert got.flag[i] == 1 and got.w[i] == L + 2 * k
+
+
+def _comp(b: bytes) -> bytes:
+    return b.translate(bytes.maketrans(b"ACGT", b"TGCA"))[::-1]
+
+
+def test_exact_reads_are_perfect_hits():
+    """Independent of the transcription: a read that occurs verbatim at genome
+    offset T > 0 has its q-gram 0 listed at T, so the perfect check at S = T
+    fires (hash.c:550-583) -- on strand 0 for a forward copy, on strand 4 for a
+    reverse-complement copy.  Both the transcription and the product filter must
+    report every one of them (offset 0 is the list terminator, Q14, so T >= 1)."""
+    L, k = 36, 2
+    genome = wl.synth_genome(606, [40_000], 0.5)
+    rng = np.random.default_rng(606)
+    T = rng.integers(1, len(genome.seq) - L, 300)
+    strand = rng.integers(0, 2, 300)
+    rd = [genome.seq[t:t + L] if s == 0 else _comp(genome.seq[t:t + L]) for t, s in zip(T, strand)]
+    n = len(rd)
+    reads = wl.Reads(b"".join(rd), (L * np.arange(n)).astype(np.uint64), np.full(n, L, np.uint16),
+                     T.astype(np.int64), strand.astype(np.uint8), np.full(n, k, np.uint8))
+    q = wl.filter_params(L, k)[0]
+    idx = wl.Index(genome, [q])
+    oidx = fo.build_index(genome.seq, q)
+    for dr, want_strand in ((0, 0), (4, 1)):
+        got = wl.run_filter(idx, genome, reads, dr, 100)
+        gp = set(zip(got.perfect_read.tolist(), got.perfect_pos.tolist()))
+        _, wperf = fo.run_strand(genome.seq, rd, [f"r{i}" for i in range(n)], L, k, dr, 100, oidx)
+        wp = {(int(x.split("\t")[0][1:]), int(x.split("\t")[1])) for x in wperf}
+        need = {(i, int(T[i])) for i in range(n) if strand[i] == want_strand}
+        assert need <= gp and need <= wp
+    idx.close()
+
+
+def test_single_substitution_reads_are_found():
+    """q-gram lemma: a 36 bp read with one substitution keeps floor(L/q) - 1 = 2
+    of its 3 non-overlapping q-grams exact (q = 11, k = 2), enough for
+    exact_tresh = 2, so its true window is among the candidates.  The first-pass
+    scan of Q12 may skip index 0 of a list, so on a random genome (lists of one
+    entry) nearly all -- not provably all -- are found."""
+    L, k = 36, 2
+    genome = wl.synth_genome(707, [60_000], 0.5)
+    rng = np.random.default_rng(707)
+    n = 400
+    T = rng.integers(1, len(genome.seq) - L - 2 * k, n)
+    rd = []
+    for t in T:
+        r = bytearray(genome.seq[t:t + L])
+        j = int(rng.integers(0, L))
+        r[j] = b"ACGT"[(b"ACGT".index(r[j]) + int(rng.integers(1, 4))) % 4]
+        rd.append(bytes(r))
+    reads = wl.Reads(b"".join(rd), (L * np.arange(n)).astype(np.uint64), np.full(n, L, np.uint16),
+                     T.astype(np.int64), np.zeros(n, np.uint8), np.full(n, k, np.uint8))
+    q = wl.filter_params(L, k)[0]
+    idx = wl.Index(genome, [q])
+    got = wl.run_filter(idx, genome, reads, 0, 1000)
+    idx.close()
+    found = set()
+    for r, p, w in zip(got.read.tolist(), got.pos.tolist(), got.w.tolist()):
+        if p <= T[r] and T[r] + L <= p + w:
+            found.add(r)
+    assert len(found) >= 0.99 * n

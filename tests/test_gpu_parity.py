@@ -177,7 +177,9 @@ def test_graph_replay(orc):
 
 
 def test_tuned_launch_shapes_keep_results(orc):
-    """Every launch shape the autotuner may pick gives the same results."""
+    """Every launch shape the autotuner may pick gives the same results: grids,
+    merged classes, and classes run concurrently on several streams, eagerly and
+    replayed from a hipGraph."""
     banner("[gpu] tuning shapes")
     from saruman_amd import tune
     from saruman_amd.batch import Context, Job, _layout, pack_codes
@@ -187,17 +189,40 @@ def test_tuned_launch_shapes_keep_results(orc):
     ctx = Context(0)
     try:
         outs = []
-        for table in (None, {"blocks_per_cu": [0] + [1] * 8, "merge_below": 3},
-                      {"blocks_per_cu": [0, 2, 1, 3, 1, 2, 1, 1, 1], "merge_below": 2}):
+        for table, graph in ((None, False),
+                             ({"blocks_per_cu": [0] * 9, "merge_below": 0, "class_streams": 1},
+                              False),
+                             ({"blocks_per_cu": [0] + [1] * 8, "merge_below": 3,
+                               "class_streams": 1}, False),
+                             ({"blocks_per_cu": [0, 2, 1, 3, 1, 2, 1, 1, 1], "merge_below": 2},
+                              False),
+                             ({"blocks_per_cu": [0] * 9, "merge_below": 0, "class_streams": 4},
+                              False),
+                             ({"blocks_per_cu": [0] * 9, "merge_below": 2, "class_streams": 2},
+                              True),
+                             ({"blocks_per_cu": [0] * 9, "merge_below": 0, "class_streams": 8},
+                              True)):
             tune.apply(ctx, table)
             job = Job(ctx)
             job.upload(0, 1, lay["cand"], pack_codes(lay["rcat"]), pack_codes(lay["wcat"]),
                        ops_total=lay["ops_total"])
-            job.run("banded")
-            outs.append(job.download())
+            job.use_graph(graph)
+            for rep in range(3 if graph else 1):     # graph: replays with syncs between
+                job.run("banded")
+                outs.append(job.download())
+                sa.align_batch(random_batch(rng, 50, L=36, k=2), kernel="banded")
             job.close()
+        want = orc.align_batch(batch)
+        assert (np.minimum(outs[0][0]["score"].astype(np.int64), batch.kvec() + 1) ==
+                np.minimum(np.asarray(want.scores, np.int64), batch.kvec() + 1)).all()
+        # transcript words each walk wrote (the rest of a slot is never written)
+        off = lay["cand"]["ops_off"].astype(np.int64)
+        used = np.zeros(len(outs[0][1]), bool)
+        for t in range(batch.n):
+            nw = (int(outs[0][0]["n_ops"][t]) + 15) // 16
+            used[off[t]:off[t] + nw] = True
         for res, ops in outs[1:]:
             assert (res == outs[0][0]).all()
-            assert (ops == outs[0][1]).all()
+            assert (ops[used] == outs[0][1][used]).all()
     finally:
         ctx.close()

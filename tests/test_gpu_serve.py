@@ -140,3 +140,23 @@ def test_over_budget_request_fails_cleanly(data):
         assert S.result(small2)[1]["status"] == 0
         with pytest.raises(RuntimeError):
             S.result(big)
+
+
+def test_device_filter_and_records_match_host(data):
+    """The server with the q-gram filter and the record formatting on the device
+    returns the same records and counts as with both on the host."""
+    print("[gpu] serve: device filter and records", flush=True)
+    genome, pool = data
+    qs = serve.qs_for_lengths(36, 250)
+    parts = slices([3000, 9000, 1500, 6000])
+    out = {}
+    for mode in (False, True):
+        with serve.Server(genome, qs, n_slots=2, chunk_value=5000, gpu_filter=mode,
+                          gpu_records=mode) as S:
+            ids = [S.submit(pool, lo, hi) for lo, hi in parts]
+            out[mode] = [S.result(i) for i in ids if S.wait(i)]
+    assert len(out[True]) == len(out[False]) == len(parts)
+    for (rd, sd), (rh, sh) in zip(out[True], out[False]):
+        assert rd == rh
+        for f in ("n_candidates", "n_hits", "n_perfect", "n_mapped"):
+            assert sd[f] == sh[f], f
