@@ -178,10 +178,12 @@ def records_leg(job, db, reads, genome, res, ops):
     rid = np.ascontiguousarray(db.read_id, np.int32)
     pos = np.ascontiguousarray(db.pos, np.int64)
     text, hits = job.records(pos, read_id=rid, prefix="r")          # first call sizes buffers
-    buf = np.empty(len(text) + 1, np.uint8)
+    pbuf = wl.Pinned(len(text) + 1)                                  # page-locked output
     t0 = time.perf_counter()
-    text, hits = job.records(pos, read_id=rid, prefix="r", out=buf)
+    got, hits = job.records(pos, read_id=rid, prefix="r", out=pbuf.array, copy=False)
     dev_ms = (time.perf_counter() - t0) * 1e3
+    text = bytes(pbuf.array[:got])
+    pbuf.free()
     fwd = np.frombuffer(reads.seq, np.uint8)
     rev = wl.revcomp_all(reads)
     roff = np.ascontiguousarray(reads.off[db.read_id], np.uint64)
@@ -205,26 +207,22 @@ def device_filter_leg(ctx, index, genome, reads, chunk, host_strands):
     checked field by field against the host filter's output."""
     from saruman_amd import workload as wl
     df = wl.DeviceFilter(ctx, index, genome)
+    preads, buf = wl.pinned_reads(reads)      # bases in page-locked memory
     try:
-        for dr in (0, 4):
-            df.run(reads, dr, chunk)
+        df.run_both(preads, chunk)
         t0 = time.perf_counter()
-        out, ms, st = [], 0.0, []
-        for dr in (0, 4):
-            out.append(df.run(reads, dr, chunk))
-            s = df.stats()
-            ms += s["device_ms"]
-            st.append(s)
+        out = df.run_both(preads, chunk)
         wall = time.perf_counter() - t0
+        st = df.stats()
     finally:
         df.close()
+        buf.free()
     fields = ("read", "pos", "flag", "w", "batch_end", "perfect_read", "perfect_pos")
     same = all(np.array_equal(getattr(h, f), getattr(d, f))
                for h, d in zip(host_strands, out) for f in fields)
-    return {"wall_s": wall, "device_ms": ms, "identical": bool(same),
-            "host_s": sum(h.seconds for h in host_strands),
-            "lists": sum(s["lists"] for s in st), "list_entries": sum(s["list_entries"] for s in st),
-            "overflow_reads": sum(s["overflow_reads"] for s in st)}
+    return {"wall_s": wall, "device_ms": st["device_ms"], "identical": bool(same),
+            "host_s": sum(h.seconds for h in host_strands), "lists": st["lists"],
+            "list_entries": st["list_entries"], "overflow_reads": st["overflow_reads"]}
 
 
 def main():

@@ -277,6 +277,10 @@ int srm_format_record(const char *name, unsigned int pos, int flag, int k, int L
 int srm_slot_rows(const srm_result *res, const uint32_t *ops, const char *read, const char *win,
                   char *R, char *G, int cap);
 
+/* page-locked host memory (fast transfers to and from the device); NULL on failure */
+void *srm_host_alloc(size_t bytes);
+void srm_host_free(void *p);
+
 /* one streaming copy of `bytes` bytes on the device (reads `bytes`, writes
  * `bytes`); used to calibrate hardware traffic counters.  ms = kernel time. */
 int srm_calibrate_copy(int device, size_t bytes, float *ms);
@@ -349,6 +353,10 @@ int srm_dfilter_create(srm_ctx *ctx, const srm_index *index, const char *genome,
 int srm_dfilter_run(srm_dfilter *df, const uint8_t *blocked, const char *reads,
                     const uint64_t *read_off, const uint16_t *read_len, const uint8_t *read_k,
                     size_t n_reads, int do_reverse, int chunk_value, srm_filter **out);
+/* both strands over one upload of the reads: the results of do_reverse 0 and 4 */
+int srm_dfilter_run2(srm_dfilter *df, const uint8_t *blocked, const char *reads,
+                     const uint64_t *read_off, const uint16_t *read_len, const uint8_t *read_k,
+                     size_t n_reads, int chunk_value, srm_filter **out_fwd, srm_filter **out_rev);
 /* last run: device time incl. transfers, lists, list entries, reads that needed HBM slots */
 int srm_dfilter_stats(const srm_dfilter *df, float *device_ms, uint64_t *n_lists,
                       uint64_t *n_list_entries, uint64_t *n_overflow);

@@ -76,7 +76,11 @@ _PROTOS = [
     ("srm_dfilter_create", C.c_int, [_vp, _vp, _vp, C.c_size_t, C.POINTER(_vp)]),
     ("srm_dfilter_run", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int, C.c_int,
                                   C.POINTER(_vp)]),
+    ("srm_dfilter_run2", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, C.c_int,
+                                   C.POINTER(_vp), C.POINTER(_vp)]),
     ("srm_dfilter_stats", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("srm_host_alloc", _vp, [C.c_size_t]),
+    ("srm_host_free", None, [_vp]),
     ("srm_dfilter_destroy", None, [_vp]),
     ("srm_synth_genome", C.c_int, [C.c_uint64, C.c_size_t, C.c_double, _vp]),
     ("srm_synth_repeats", C.c_longlong, [C.c_uint64, _vp, C.c_size_t, C.c_int, C.c_int, C.c_int,

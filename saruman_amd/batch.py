@@ -190,11 +190,14 @@ class Job:
         check(self.lib.srm_job_download(self.h, ptr(res), ptr(ops), ops.size), "srm_job_download")
         return res, ops
 
-    def records(self, pos, read_id=None, names=None, name_stride=60, prefix="", out=None):
+    def records(self, pos, read_id=None, names=None, name_stride=60, prefix="", out=None,
+                copy=True):
         """TSV records of every hit of the last run, formatted on the device
         (srm_job_records).  names: None (prefix + decimal read id) or a bytes
         table of name_stride-byte NUL-terminated names indexed by read_id.
-        out: optional preallocated uint8 buffer.  Returns (text bytes, hits)."""
+        out: optional preallocated uint8 buffer (page-locked memory makes the
+        copy-out fast).  Returns (text bytes, hits), or (byte count, hits) with
+        copy=False (the text is then in out)."""
         pos = np.ascontiguousarray(pos, np.int64)
         rid = None if read_id is None else np.ascontiguousarray(read_id, np.int32)
         nb = None if names is None else np.frombuffer(names, np.uint8)
@@ -213,7 +216,7 @@ class Job:
                                            n_names, pfx, ptr(out), out.size, C.byref(hits))
             if got < 0:
                 check(int(got), "srm_job_records")
-        return out[:got].tobytes(), int(hits.value)
+        return (out[:got].tobytes() if copy else int(got)), int(hits.value)
 
     def close(self):
         if self.h:

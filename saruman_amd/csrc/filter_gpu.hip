@@ -267,9 +267,12 @@ struct srm_dfilter {
 
 namespace {
 
+// one or both strands (strands: bit 0 forward, bit 1 reverse) over one upload of
+// the reads
 int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
                const uint64_t *read_off, const uint16_t *read_len, const uint8_t *read_k,
-               size_t n_reads, int do_reverse, int chunk_value, srm_filter **out)
+               size_t n_reads, int strands, int chunk_value, srm_filter **out_fwd,
+               srm_filter **out_rev)
 {
     ReadPlan plan;
     std::string err;
@@ -282,7 +285,9 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
     const auto &list_base = plan.list_base;
     const uint64_t nl = list_base[n_reads];
     const uint64_t rbytes = plan.read_bytes;
-    G->last_lists = nl;
+    G->last_lists = 0;
+    G->last_flat = 0;
+    G->last_overflow = 0;
     FL_TRY(hipSetDevice(G->device));
     FL_TRY(hipEventRecord(G->ev[0], G->st));
     FL_RC(G->upload(G->par, dpar.data(), dpar.size() * sizeof(DParam)));
@@ -305,76 +310,81 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
     a.rlen = G->rlen.as<uint16_t>();
     a.pid = G->pid.as<uint16_t>();
     a.n_reads = (uint32_t)n_reads;
-    a.dr = do_reverse;
     a.cv = (uint32_t)chunk_value;
     a.list_base = G->list_base.as<uint32_t>();
     a.list_cnt = G->list_cnt.as<uint32_t>();
     a.list_start = G->list_start.as<uint32_t>();
-    const unsigned grid = (unsigned)((n_reads + FB - 1) / FB);
-    FL_TRY(hipMemsetAsync(a.list_cnt + nl, 0, 4, G->st));
-    hipLaunchKernelGGL(fl_count, dim3(grid), dim3(FB), 0, G->st, a);
-    FL_TRY(hipGetLastError());
-    size_t tmp = 0;
-    FL_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, a.list_cnt, G->list_start.as<uint32_t>(),
-                                            (int)(nl + 1), G->st));
-    FL_RC(G->cub_tmp.ensure(tmp));
-    FL_TRY(hipcub::DeviceScan::ExclusiveSum(G->cub_tmp.p, tmp, a.list_cnt,
-                                            G->list_start.as<uint32_t>(), (int)(nl + 1), G->st));
-    uint32_t total = 0;
-    FL_TRY(hipMemcpyAsync(&total, G->list_start.as<uint32_t>() + nl, 4, hipMemcpyDeviceToHost,
-                          G->st));
-    FL_TRY(hipStreamSynchronize(G->st));
-    G->last_flat = total;
-    FL_RC(G->flat.ensure((size_t)total * 4 + 4));
-    a.flat = G->flat.as<uint32_t>();
-    hipLaunchKernelGGL(fl_fill, dim3(grid), dim3(FB), 0, G->st, a);
-    FL_TRY(hipGetLastError());
+    for (int dr : {0, 4}) {
+        if (!(strands & (dr ? 2 : 1))) continue;
+        a.dr = dr;
+        G->last_lists += nl;
+        const unsigned grid = (unsigned)((n_reads + FB - 1) / FB);
+        FL_TRY(hipMemsetAsync(a.list_cnt + nl, 0, 4, G->st));
+        hipLaunchKernelGGL(fl_count, dim3(grid), dim3(FB), 0, G->st, a);
+        FL_TRY(hipGetLastError());
+        size_t tmp = 0;
+        FL_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, a.list_cnt, G->list_start.as<uint32_t>(),
+                                                (int)(nl + 1), G->st));
+        FL_RC(G->cub_tmp.ensure(tmp));
+        FL_TRY(hipcub::DeviceScan::ExclusiveSum(G->cub_tmp.p, tmp, a.list_cnt,
+                                                G->list_start.as<uint32_t>(), (int)(nl + 1), G->st));
+        uint32_t total = 0;
+        FL_TRY(hipMemcpyAsync(&total, G->list_start.as<uint32_t>() + nl, 4, hipMemcpyDeviceToHost,
+                              G->st));
+        FL_TRY(hipStreamSynchronize(G->st));
+        G->last_flat += total;
+        FL_RC(G->flat.ensure((size_t)total * 4 + 4));
+        a.flat = G->flat.as<uint32_t>();
+        hipLaunchKernelGGL(fl_fill, dim3(grid), dim3(FB), 0, G->st, a);
+        FL_TRY(hipGetLastError());
 
-    std::vector<uint32_t> meta;
-    std::vector<int32_t> pos, perf;
-    std::vector<int8_t> flag;
-    FL_RC(G->pair(a, false, nullptr, (uint32_t)n_reads, ECAP, PCAP, meta, pos, flag, perf));
-    // reads that outgrew the LDS slots run again with slots in HBM
-    std::vector<uint32_t> over;
-    for (size_t r = 0; r < n_reads; ++r)
-        if (meta[3 * r + 2] == OVERFLOW_MARK) over.push_back((uint32_t)r);
-    G->last_overflow = over.size();
-    // (65534: a count word of 0xFFFF | 0xFFFF << 16 would read as the overflow mark)
-    for (uint32_t cap : {1024u, 65534u}) {
-        if (over.empty()) break;
-        const size_t per = std::max<size_t>(1, ((size_t)256 << 20) / ((size_t)cap * 9));
-        std::vector<uint32_t> still;
-        for (size_t b = 0; b < over.size(); b += per) {
-            const uint32_t m = (uint32_t)std::min(per, over.size() - b);
-            std::vector<uint32_t> m2;
-            std::vector<int32_t> p2, q2;
-            std::vector<int8_t> f2;
-            FL_RC(G->pair(a, true, over.data() + b, m, cap, cap, m2, p2, f2, q2));
-            const uint32_t es = (uint32_t)pos.size(), ps = (uint32_t)perf.size();
-            pos.insert(pos.end(), p2.begin(), p2.end());
-            flag.insert(flag.end(), f2.begin(), f2.end());
-            perf.insert(perf.end(), q2.begin(), q2.end());
-            for (uint32_t t = 0; t < m; ++t) {
-                const uint32_t r = over[b + t];
-                if (m2[3 * t + 2] == OVERFLOW_MARK) {
-                    still.push_back(r);
-                    continue;
+        std::vector<uint32_t> meta;
+        std::vector<int32_t> pos, perf;
+        std::vector<int8_t> flag;
+        FL_RC(G->pair(a, false, nullptr, (uint32_t)n_reads, ECAP, PCAP, meta, pos, flag, perf));
+        // reads that outgrew the LDS slots run again with slots in HBM
+        std::vector<uint32_t> over;
+        for (size_t r = 0; r < n_reads; ++r)
+            if (meta[3 * r + 2] == OVERFLOW_MARK) over.push_back((uint32_t)r);
+        G->last_overflow += over.size();
+        // (65534: a count word of 0xFFFF | 0xFFFF << 16 would read as the overflow mark)
+        for (uint32_t cap : {1024u, 65534u}) {
+            if (over.empty()) break;
+            const size_t per = std::max<size_t>(1, ((size_t)256 << 20) / ((size_t)cap * 9));
+            std::vector<uint32_t> still;
+            for (size_t b = 0; b < over.size(); b += per) {
+                const uint32_t m = (uint32_t)std::min(per, over.size() - b);
+                std::vector<uint32_t> m2;
+                std::vector<int32_t> p2, q2;
+                std::vector<int8_t> f2;
+                FL_RC(G->pair(a, true, over.data() + b, m, cap, cap, m2, p2, f2, q2));
+                const uint32_t es = (uint32_t)pos.size(), ps = (uint32_t)perf.size();
+                pos.insert(pos.end(), p2.begin(), p2.end());
+                flag.insert(flag.end(), f2.begin(), f2.end());
+                perf.insert(perf.end(), q2.begin(), q2.end());
+                for (uint32_t t = 0; t < m; ++t) {
+                    const uint32_t r = over[b + t];
+                    if (m2[3 * t + 2] == OVERFLOW_MARK) {
+                        still.push_back(r);
+                        continue;
+                    }
+                    meta[3 * r + 0] = es + m2[3 * t + 0];
+                    meta[3 * r + 1] = ps + m2[3 * t + 1];
+                    meta[3 * r + 2] = m2[3 * t + 2];
                 }
-                meta[3 * r + 0] = es + m2[3 * t + 0];
-                meta[3 * r + 1] = ps + m2[3 * t + 1];
-                meta[3 * r + 2] = m2[3 * t + 2];
             }
+            over.swap(still);
         }
-        over.swap(still);
+        if (!over.empty())
+            return srm::set_error(SRM_ENOMEM, "read %u holds more than 65534 filter entries",
+                                  over[0]);
+        FL_TRY(hipEventRecord(G->ev[1], G->st));
+        FL_TRY(hipEventSynchronize(G->ev[1]));
+        FL_TRY(hipEventElapsedTime(&G->last_ms, G->ev[0], G->ev[1]));
+        srm_filter *F = batch_pass(MetaAcc{meta, pos, flag, perf}, n_reads, read_len, read_k,
+                                   G->glen, blocked, chunk_value);
+        *(dr ? out_rev : out_fwd) = F;
     }
-    if (!over.empty())
-        return srm::set_error(SRM_ENOMEM, "read %u holds more than 65534 filter entries",
-                              over[0]);
-    FL_TRY(hipEventRecord(G->ev[1], G->st));
-    FL_TRY(hipEventSynchronize(G->ev[1]));
-    FL_TRY(hipEventElapsedTime(&G->last_ms, G->ev[0], G->ev[1]));
-    *out = batch_pass(MetaAcc{meta, pos, flag, perf}, n_reads, read_len, read_k, G->glen, blocked,
-                      chunk_value);
     return SRM_OK;
 }
 
@@ -450,8 +460,40 @@ int srm_dfilter_run(srm_dfilter *G, const uint8_t *blocked, const char *reads,
         *out = new srm_filter();
         return SRM_OK;
     }
-    return run_device(G, blocked, reads, read_off, read_len, read_k, n_reads, do_reverse,
-                      chunk_value, out);
+    srm_filter *F = nullptr;
+    const int rc = run_device(G, blocked, reads, read_off, read_len, read_k, n_reads,
+                              do_reverse ? 2 : 1, chunk_value, &F, &F);
+    if (rc != SRM_OK) {
+        delete F;
+        return rc;
+    }
+    *out = F;
+    return SRM_OK;
+}
+
+int srm_dfilter_run2(srm_dfilter *G, const uint8_t *blocked, const char *reads,
+                     const uint64_t *read_off, const uint16_t *read_len, const uint8_t *read_k,
+                     size_t n_reads, int chunk_value, srm_filter **out_fwd, srm_filter **out_rev)
+{
+    *out_fwd = nullptr;
+    *out_rev = nullptr;
+    if (!G || chunk_value < 1) return srm::set_error(SRM_EINVAL, "bad filter arguments");
+    if (n_reads == 0) {
+        *out_fwd = new srm_filter();
+        *out_rev = new srm_filter();
+        return SRM_OK;
+    }
+    srm_filter *Ff = nullptr, *Fr = nullptr;
+    const int rc = run_device(G, blocked, reads, read_off, read_len, read_k, n_reads, 3,
+                              chunk_value, &Ff, &Fr);
+    if (rc != SRM_OK) {
+        delete Ff;
+        delete Fr;
+        return rc;
+    }
+    *out_fwd = Ff;
+    *out_rev = Fr;
+    return SRM_OK;
 }
 
 int srm_dfilter_stats(const srm_dfilter *G, float *device_ms, uint64_t *n_lists,

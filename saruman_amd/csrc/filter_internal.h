@@ -144,7 +144,10 @@ srm_filter *batch_pass(const Acc &ro, size_t n_reads, const uint16_t *read_len,
     std::vector<size_t> next_with(n_reads + 1, n_reads);   // next read with entries
     for (size_t r = n_reads; r-- > 0;)
         next_with[r] = (r + 1 < n_reads && ro.n_ent(r + 1) != 0) ? r + 1 : next_with[r + 1];
-    std::vector<std::vector<uint8_t>> live(n_reads);
+    // live flag of entry i of read r at live[base[r] + i]
+    std::vector<size_t> base(n_reads + 1, 0);
+    for (size_t r = 0; r < n_reads; ++r) base[r + 1] = base[r] + ro.n_ent(r);
+    std::vector<uint8_t> live(base[n_reads], 0);
     std::vector<size_t> sticky;
     uint64_t queue = 0;
     size_t pending_lo = 0;
@@ -153,8 +156,7 @@ srm_filter *batch_pass(const Acc &ro, size_t n_reads, const uint16_t *read_len,
         for (size_t r = lo; r < hi; ++r) {
             const size_t ne = ro.n_ent(r);
             if (ne == 0) continue;
-            live[r].resize(ne);
-            for (size_t i = 0; i < ne; ++i) live[r][i] = ro.pos_at(r, i) > 0;
+            for (size_t i = 0; i < ne; ++i) live[base[r] + i] = ro.pos_at(r, i) > 0;
             order.push_back(r);
         }
         int start_offset = 0, end_offset = 0;
@@ -171,7 +173,7 @@ srm_filter *batch_pass(const Acc &ro, size_t n_reads, const uint16_t *read_len,
             bool keep = false;
             const size_t ne = ro.n_ent(r);
             for (size_t i = 0; i < ne; ++i) {
-                if (live[r][i]) {
+                if (live[base[r] + i]) {
                     const int32_t pos = ro.pos_at(r, i);
                     const int flag = ro.flag_at(r, i);
                     switch (flag) {   // hash.c:938-979
@@ -192,7 +194,7 @@ srm_filter *batch_pass(const Acc &ro, size_t n_reads, const uint16_t *read_len,
                     F->c_pos.push_back(pos);
                     F->c_flag.push_back((uint8_t)flag);
                     F->c_w.push_back((uint16_t)w);
-                    live[r][i] = 0;        // :1077 negated
+                    live[base[r] + i] = 0;   // :1077 negated
                 }
                 end_offset = 0;
                 start_offset = 0;
@@ -232,19 +234,23 @@ struct ReadPlan {
 inline int plan_reads(const srm_index *ix, const uint64_t *read_off, const uint16_t *read_len,
                       const uint8_t *read_k, size_t n_reads, ReadPlan &P, std::string &err)
 {
-    std::map<std::pair<int, int>, int> pmap;
+    // parameter set of each (length, k): a table over the lengths present and k
+    // 0..255, filled on first use
+    int maxlen = 0;
+    for (size_t i = 0; i < n_reads; ++i) maxlen = std::max(maxlen, (int)read_len[i]);
+    std::vector<int32_t> slot(((size_t)maxlen + 1) * 256, -1);
     P.pid.resize(n_reads);
     P.list_base.resize(n_reads + 1);
     uint64_t nl = 0;
     for (size_t i = 0; i < n_reads; ++i) {
-        auto key = std::make_pair((int)read_len[i], (int)read_k[i]);
-        auto it = pmap.find(key);
-        if (it == pmap.end()) {
-            if (key.first < 1) {
+        const int L = read_len[i], k = read_k[i];
+        int32_t &id = slot[(size_t)L * 256 + (size_t)k];
+        if (id < 0) {
+            if (L < 1) {
                 err = "a read has length 0";
                 return SRM_EINVAL;
             }
-            P.params.push_back(derive(key.first, key.second));
+            P.params.push_back(derive(L, k));
             if (!ix->get(P.params.back().q)) {
                 err = "the index lacks q = " + std::to_string(P.params.back().q);
                 return SRM_EINVAL;
@@ -253,11 +259,11 @@ inline int plan_reads(const srm_index *ix, const uint64_t *read_off, const uint1
                 err = "too many (length, k) classes";
                 return SRM_EINVAL;
             }
-            it = pmap.emplace(key, (int)P.params.size() - 1).first;
+            id = (int32_t)P.params.size() - 1;
         }
-        P.pid[i] = (uint16_t)it->second;
+        P.pid[i] = (uint16_t)id;
         P.list_base[i] = (uint32_t)nl;
-        nl += (uint64_t)P.params[(size_t)it->second].n;
+        nl += (uint64_t)P.params[(size_t)id].n;
         P.read_bytes = std::max<uint64_t>(P.read_bytes, read_off[i] + read_len[i]);
         if (nl >= ((uint64_t)1 << 31)) {
             err = "too many q-gram lists in one call";

@@ -40,8 +40,8 @@ struct BandArgs {
     uint32_t *queue;                // tile counter (zeroed before the launch)
     const uint32_t *read_words;
     const uint32_t *win_words;
-    uint32_t *scratch;              // per wave: 8*NW * ceil((maxL + 1) / 8) * 64 words
-    size_t scratch_words_per_wave;
+    uint32_t *scratch;              // band nibbles of every tile of the class
+    const uint64_t *tile_scratch;   // [ntiles] word offset of each tile's band in scratch
     srm_result *res;
     uint32_t *ops;
 };
@@ -74,6 +74,7 @@ hipError_t launch_rec_write(const RecArgs &a, hipStream_t st);
 hipError_t rec_scan(void *tmp, size_t &tmp_bytes, const uint64_t *len, uint64_t *off, int n,
                     hipStream_t st);
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
+hipError_t launch_trace(const BandArgs &a, int nw, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);

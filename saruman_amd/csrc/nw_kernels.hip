@@ -168,12 +168,11 @@ __global__ __launch_bounds__(256) void plain_kernel(PlainArgs a)
 // ---------------------------------------------------------------------------
 // banded persistent kernel
 // ---------------------------------------------------------------------------
-// The wave's scratch holds the band values as nibbles in blocks of 8 columns:
+// A tile's band scratch holds the band values as nibbles in blocks of 8 columns:
 // word (q, i) of lane l sits at ((q * BMAX + i) * 64 + l) and holds band cell
 // (diagonal) i of columns 8q .. 8q+7 (column 8q + j in bits 4j..4j+3).  The
 // forward pass fills the BMAX words of a block every 8 columns with coalesced
-// 256-byte wave stores at immediate offsets; the traceback keeps the words of
-// five diagonals in three blocks around its walk in registers.
+// 256-byte wave stores at immediate offsets; trace_kernel reads them back.
 template <int BMAX>
 __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, int i, int q,
                                               int lane)
@@ -185,7 +184,7 @@ __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, 
 // budget is 512 / waves VGPRs
 template <int NW>
 struct BandWaves {
-    static constexpr int value = NW <= 1 ? 5 : NW <= 3 ? 4 : NW <= 5 ? 3 : 2;
+    static constexpr int value = NW <= 1 ? 8 : NW <= 2 ? 5 : NW <= 3 ? 4 : NW <= 5 ? 3 : 2;
 };
 
 template <int NW, bool UNIT>
@@ -195,8 +194,6 @@ void band_kernel(BandArgs a)
     constexpr int BMAX = NW * 8;            // band cells per column held in VGPRs
     constexpr int WW = (BMAX + 15) / 16;    // code words of the sliding window
     const int lane = threadIdx.x & 63;
-    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    uint32_t *__restrict__ scr = a.scratch + (size_t)wave * a.scratch_words_per_wave;
     const int ma = a.ma, mm = a.mm;
 
     for (;;) {
@@ -205,6 +202,7 @@ void band_kernel(BandArgs a)
         tile = __builtin_amdgcn_readfirstlane(tile);
         if (tile >= a.ntiles) break;
         const uint4 td = a.tiles[tile];     // x: perm start, y: count, z: max L, w: max k
+        uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
         const bool active = (uint32_t)lane < td.y;
         const uint32_t t = active ? a.perm[td.x + lane] : 0u;
         srm_cand cd = a.cand[t];
@@ -342,176 +340,129 @@ void band_kernel(BandArgs a)
         }
         if (active) column(L, true);
 
-        // ---- score (cuda-functions.cu:495-497)
-        int score = K;
+        // ---- score (cuda-functions.cu:495-497); the walk runs in trace_kernel
         if (active) {
+            int score = K;
             const int iend = w - L - dlo;
             if (iend >= 0 && iend < BMAX) {
                 const uint32_t wd0 = band_word<BMAX>(scr, iend, L >> 3, lane);
                 score = imin((int)((wd0 >> ((L & 7) * 4)) & 15u), K);
             }
-        }
-
-        // ---- traceback (cuda-functions.cu:504-534, App. A.3) in phases of 8 steps.
-        // A walk moves at most one column and one row per step, so at the start of a
-        // phase every lane loads what its next 8 steps can touch, one phase ahead:
-        //  - band words of diagonals bi-2..bi+2 in blocks bq, bq-1, bq-2 (BA, BB, BC);
-        //    entering block bq-1 rotates them and fetches block bq-3;
-        //  - 32 read codes and 32 window codes covering the next phase.
-        // The tiles hold candidates of equal L mod 8, so diagonal walks cross blocks
-        // together and the rotation loads of all lanes overlap the next phase; an
-        // indel that leaves the cached diagonals reloads at once (rare).
-        bool walking = active && score <= k;
-        const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
-        uint32_t *__restrict__ ops = a.ops + (walking ? cd.ops_off : 0u);
-        const int cap = w + k;
-        const int64_t gorg = (int64_t)cd.win_off;
-        int row = w, col = L, n = 0;
-        uint32_t opw = 0;
-        uint32_t BA[5], BB[5], BC[5];
-        int bq = -8, bi = 0, vlo = 1, vhi = 0;     // nothing cached yet
-        auto load_block = [&](uint32_t (&X)[5], int q, int i) {
-#pragma unroll
-            for (int d = 0; d < 5; ++d) {
-                const int ii = i - 2 + d;
-                X[d] = (q >= 0 && ii >= 0 && ii < BMAX) ? band_word<BMAX>(scr, ii, q, lane)
-                                                         : 0xFFFFFFFFu;
-            }
-        };
-        auto load_all = [&](int q, int i) {
-            load_block(BA, q, i);
-            load_block(BB, q - 1, i);
-            load_block(BC, q - 2, i);
-            bq = q;
-            bi = i;
-            vlo = i - 2;
-            vhi = i + 2;
-        };
-        // code windows: 2 words from absolute code index 16 * base.  A phase that
-        // starts at column c reads codes c-9 .. c-1; its window is fetched at the
-        // previous phase's start c' <= c + 8 from c' - 17, so 32 codes cover it.
-        uint32_t RC[2] = {0u, 0u}, GC[2] = {0u, 0u}, RN[2] = {0u, 0u}, GN[2] = {0u, 0u};
-        // (word indices: code arrays stay below 2^35 codes)
-        int rcb = 0, gcb = 0, rnb = 0, gnb = 0;
-        auto load_codes = [&](uint32_t (&X)[2], int &xb, const uint32_t *words, int64_t abs_lo) {
-            xb = (int)(abs_lo >> 4);
-            X[0] = words[xb];
-            X[1] = words[xb + 1];
-        };
-        auto code2 = [&](const uint32_t (&X)[2], int xb, int64_t abs_idx) -> int {
-            const int rel = (int)(abs_idx - 16 * (int64_t)xb);   // 0..31
-            const uint32_t wv = rel < 16 ? X[0] : X[1];
-            return (int)((wv >> ((rel & 15) * 2)) & 3u);
-        };
-        if (walking) {
-            load_all(L >> 3, w - L - dlo);
-            load_codes(RC, rcb, a.read_words, rbase + imax(L - 9, 0));
-            load_codes(GC, gcb, a.win_words, gorg + imax(w - 9, 0));
-            load_codes(RN, rnb, a.read_words, rbase + imax(L - 17, 0));
-            load_codes(GN, gnb, a.win_words, gorg + imax(w - 17, 0));
-        }
-        bool first = true;
-        while (__any(walking)) {
-            if (walking && !first) {
-                // phase start: this phase's codes were fetched one phase ago
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    RC[j] = RN[j];
-                    GC[j] = GN[j];
-                }
-                rcb = rnb;
-                gcb = gnb;
-                load_codes(RN, rnb, a.read_words, rbase + imax(col - 17, 0));
-                load_codes(GN, gnb, a.win_words, gorg + imax(row - 17, 0));
-                const int q = col >> 3, i = row - col - dlo;
-                if (q == bq - 1 && i == bi && vlo == i - 2 && vhi == i + 2) {
-#pragma unroll
-                    for (int d = 0; d < 5; ++d) {
-                        BA[d] = BB[d];
-                        BB[d] = BC[d];
-                    }
-                    load_block(BC, q - 2, i);
-                    bq = q;
-                } else if (!(q == bq && i == bi && vlo == i - 2 && vhi == i + 2)) {
-                    load_all(q, i);
-                }
-            }
-            first = false;
-            for (int stp = 0; stp < 8; ++stp) {
-                if (!walking) break;
-                if (stop_row0 && row == 0) {
-                    walking = false;
-                    break;
-                }
-                const int i = row - col - dlo;
-                // an indel moves the walk by one diagonal: shift the cached words
-                if (i == bi - 1) {
-#pragma unroll
-                    for (int d = 4; d > 0; --d) {
-                        BA[d] = BA[d - 1];
-                        BB[d] = BB[d - 1];
-                        BC[d] = BC[d - 1];
-                    }
-                    bi = i;
-                    vhi = imin(vhi, i + 2);
-                } else if (i == bi + 1) {
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) {
-                        BA[d] = BA[d + 1];
-                        BB[d] = BB[d + 1];
-                        BC[d] = BC[d + 1];
-                    }
-                    bi = i;
-                    vlo = imax(vlo, i - 2);
-                }
-                const int q = col >> 3;
-                const int qd = col > 0 ? (col - 1) >> 3 : q;   // block of column col - 1
-                if (i != bi || i - 1 < vlo || i + 1 > vhi || bq - qd > 2 || q > bq) load_all(q, i);
-                const int o = bq - q, od = bq - qd;           // 0, 1 or 2: BA, BB, BC
-                const uint32_t Wm = o == 0 ? BA[1] : o == 1 ? BB[1] : BC[1];
-                const uint32_t W0 = o == 0 ? BA[2] : o == 1 ? BB[2] : BC[2];
-                const uint32_t D0 = od == 0 ? BA[2] : od == 1 ? BB[2] : BC[2];
-                const uint32_t Dp = od == 0 ? BA[3] : od == 1 ? BB[3] : BC[3];
-                const uint32_t sh = (uint32_t)(col & 7) * 4u;
-                const uint32_t psh = (uint32_t)((col - 1) & 7) * 4u;
-                // a nibble of 15 outside the band reads as K (never equal to v <= k)
-                const int vv = (int)((W0 >> sh) & 15u);
-                int Dg, Lf;
-                if (col != 0) {
-                    Dg = row >= 1 ? imin((int)((D0 >> psh) & 15u), K) : SRM_FOREIGN;
-                    Lf = imin((int)((Dp >> psh) & 15u), K);
-                } else {
-                    Dg = SRM_COL0;
-                    Lf = SRM_COL0;
-                }
-                const int Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
-                const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
-                const bool eq = code2(GC, gcb, gorg + ht) == code2(RC, rcb, rbase + wd);
-                const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
-                if (op == 3 || n >= cap) {
-                    walking = false;
-                    break;
-                }
-                opw |= (uint32_t)op << ((n & 15) * 2);
-                ++n;
-                if ((n & 15) == 0) {
-                    ops[(n >> 4) - 1] = opw;
-                    opw = 0;
-                }
-                row -= (op != 2);
-                col -= (op != 1);
-            }
-        }
-        if (active) {
             srm_result r;
             r.score = (int16_t)score;
-            r.n_ops = (uint16_t)n;
-            r.end_row = (uint16_t)row;
-            r.end_col = (uint16_t)col;
-            if (score <= k && (n & 15)) ops[n >> 4] = opw;
+            r.n_ops = 0;
+            r.end_row = (uint16_t)w;
+            r.end_col = (uint16_t)L;
             a.res[t] = r;
         }
     }
+}
+
+// word (q, i) of a lane's band, 0xFFFFFFFF (nibbles read as K) outside it
+template <int BMAX>
+__device__ __forceinline__ uint32_t band_word_or_k(const uint32_t *__restrict__ scr, int i, int q,
+                                                  int lane)
+{
+    return (i >= 0 && i < BMAX && q >= 0) ? band_word<BMAX>(scr, i, q, lane) : 0xFFFFFFFFu;
+}
+
+// 16 codes of a sequence starting at a multiple of 16 at or below idx
+struct CodeCache {
+    int64_t base = INT64_MIN;
+    uint32_t word = 0;
+    __device__ __forceinline__ int at(const uint32_t *__restrict__ words, int64_t org, int idx)
+    {
+        const int64_t b = (int64_t)(idx & ~15);
+        if (b != base) {
+            base = b;
+            word = fetch16(words, org + b);
+        }
+        return (int)((word >> ((idx & 15) * 2)) & 3u);
+    }
+};
+
+// Traceback of every hit of a band class (cuda-functions.cu:504-534, App. A.3),
+// one wave per tile of band_kernel, reading the band the forward pass left in
+// the tile's scratch.  Few registers and many resident waves: the walk's reloads
+// (one block of 8 columns per reload on a diagonal walk) wait for memory while
+// other waves run.
+template <int NW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+void trace_kernel(BandArgs a)
+{
+    constexpr int BMAX = NW * 8;
+    const int lane = threadIdx.x & 63;
+    const uint32_t tile = (uint32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (tile >= a.ntiles) return;
+    const uint4 td = a.tiles[tile];
+    if ((uint32_t)lane >= td.y) return;
+    const uint32_t t = a.perm[td.x + lane];
+    const srm_cand cd = a.cand[t];
+    srm_result r = a.res[t];
+    const int k = cd.k, K = k + 1;
+    if (r.score > k) return;
+    const int L = cd.L, w = cd.w;
+    const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
+    const int dlo = -k;
+    const int ma = a.ma, mm = a.mm;
+    const uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
+    const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
+    uint32_t *__restrict__ ops = a.ops + cd.ops_off;
+    const int64_t rbase = (int64_t)cd.read_off;
+    const int cap = w + k;
+    int row = w, col = L, n = 0;
+    uint32_t opw = 0;
+    // cached words of diagonals ci-1, ci, ci+1 in column block cq (Wm, W0, Wp) and
+    // of ci, ci+1 in block cq-1 (P0, Pp)
+    int ci = -1, cq = -1;
+    uint32_t Wm = 0, W0 = 0, Wp = 0, P0 = 0, Pp = 0;
+    CodeCache rcc, gcc;
+    for (;;) {
+        if (stop_row0 && row == 0) break;
+        const int i = row - col - dlo;
+        const int q = col >> 3;
+        if (i != ci || q != cq) {
+            W0 = band_word_or_k<BMAX>(scr, i, q, lane);
+            Wm = band_word_or_k<BMAX>(scr, i - 1, q, lane);
+            Wp = band_word_or_k<BMAX>(scr, i + 1, q, lane);
+            P0 = band_word_or_k<BMAX>(scr, i, q - 1, lane);
+            Pp = band_word_or_k<BMAX>(scr, i + 1, q - 1, lane);
+            ci = i;
+            cq = q;
+        }
+        const uint32_t sh = (uint32_t)(col & 7) * 4u;
+        // a nibble of 15 outside the band reads as K (never equal to v <= k)
+        const int vv = (int)((W0 >> sh) & 15u);
+        int Dg, Lf;
+        if (col != 0) {
+            const bool in = (col & 7) != 0;
+            const uint32_t dgw = in ? W0 : P0, lfw = in ? Wp : Pp;
+            const uint32_t psh = in ? sh - 4u : 28u;
+            Dg = row >= 1 ? imin((int)((dgw >> psh) & 15u), K) : SRM_FOREIGN;
+            Lf = imin((int)((lfw >> psh) & 15u), K);
+        } else {
+            Dg = SRM_COL0;
+            Lf = SRM_COL0;
+        }
+        const int Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
+        const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
+        const bool eq = gcc.at(a.win_words, (int64_t)cd.win_off, ht) == rcc.at(a.read_words, rbase, wd);
+        const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
+        if (op == 3 || n >= cap) break;
+        opw |= (uint32_t)op << ((n & 15) * 2);
+        ++n;
+        if ((n & 15) == 0) {
+            ops[(n >> 4) - 1] = opw;
+            opw = 0;
+        }
+        row -= (op != 2);
+        col -= (op != 1);
+    }
+    if (n & 15) ops[n >> 4] = opw;
+    r.n_ops = (uint16_t)n;
+    r.end_row = (uint16_t)row;
+    r.end_col = (uint16_t)col;
+    a.res[t] = r;
 }
 
 // streaming copy of n 16-byte words (calibrates the HBM traffic counters)
@@ -602,6 +553,31 @@ hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStre
     case 6: return launch_band_nw<6>(a, unit, blocks, st);
     case 7: return launch_band_nw<7>(a, unit, blocks, st);
     case 8: return launch_band_nw<8>(a, unit, blocks, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NW>
+static hipError_t launch_trace_nw(const BandArgs &a, hipStream_t st)
+{
+    dim3 block(256), grid((a.ntiles + 3) / 4);
+    hipLaunchKernelGGL((trace_kernel<NW>), grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+// one wave per tile of the class
+hipError_t launch_trace(const BandArgs &a, int nw, hipStream_t st)
+{
+    if (a.ntiles == 0) return hipSuccess;
+    switch (nw) {
+    case 1: return launch_trace_nw<1>(a, st);
+    case 2: return launch_trace_nw<2>(a, st);
+    case 3: return launch_trace_nw<3>(a, st);
+    case 4: return launch_trace_nw<4>(a, st);
+    case 5: return launch_trace_nw<5>(a, st);
+    case 6: return launch_trace_nw<6>(a, st);
+    case 7: return launch_trace_nw<7>(a, st);
+    case 8: return launch_trace_nw<8>(a, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -128,7 +128,6 @@ struct BandClass {
     int nw = 0;
     uint32_t tile_begin = 0, ntiles = 0;
     int maxL = 0;
-    size_t scratch_off = 0;   // words into the job's band scratch
     int blocks = 0;
 };
 struct PlainChunk {
@@ -145,7 +144,7 @@ struct srm_job {
     bool resident_genome = false;
     // device buffers
     DevBuf d_cand, d_read_words, d_win_words, d_read_bytes, d_win_bytes;
-    DevBuf d_perm, d_tiles, d_queue, d_scratch;
+    DevBuf d_perm, d_tiles, d_tile_scratch, d_queue, d_scratch;
     DevBuf d_idx_rest, d_idx_all, d_idx_bytes;      // plain index lists
     DevBuf d_off_rest, d_off_all, d_off_bytes;      // matrix offsets (per list, chunk-relative)
     DevBuf d_mat, d_res, d_ops;
@@ -185,11 +184,8 @@ inline int band_cells(const srm_cand &c)
     return hi + k + 1;
 }
 
-// words of one wave's band scratch: 8*NW diagonals x 8-column blocks x 64 lanes
-inline size_t band_scratch_words(const BandClass &bc)
-{
-    return (size_t)bc.nw * 8 * (size_t)((bc.maxL + 8) / 8) * 64;
-}
+// words of one tile's band scratch: 8*NW diagonals x 8-column blocks x 64 lanes
+inline size_t tile_scratch_words(int nw, int maxL) { return (size_t)nw * 8 * ((maxL + 8) / 8) * 64; }
 
 void destroy_graph(srm_job *j)
 {
@@ -276,12 +272,13 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.win_words = j->resident_genome
                               ? ctx->genome_ptr()
                               : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
-            a.scratch = (uint32_t *)j->d_scratch.p + bc.scratch_off;
-            a.scratch_words_per_wave = band_scratch_words(bc);
+            a.scratch = (uint32_t *)j->d_scratch.p;
+            a.tile_scratch = (const uint64_t *)j->d_tile_scratch.p + bc.tile_begin;
             a.res = (srm_result *)j->d_res.p;
             a.ops = (uint32_t *)j->d_ops.p;
             if (timed) HIP_TRY(hipEventRecord(j->cs[c], cst));
             HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, cst));
+            HIP_TRY(srm::launch_trace(a, bc.nw, cst));
             if (timed) HIP_TRY(hipEventRecord(j->ce[c], cst));
         }
         if (S > 1)
@@ -361,6 +358,21 @@ int srm_device_count(void)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+void *srm_host_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        fail(SRM_ENOMEM, "hipHostMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+void srm_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
 }
 
 int srm_device_sync(int device)
@@ -486,7 +498,8 @@ void srm_job_destroy(srm_job *j)
     (void)hipStreamSynchronize(c->stream);
     destroy_graph(j);
     DevBuf *bufs[] = {&j->d_cand, &j->d_read_words, &j->d_win_words, &j->d_read_bytes,
-                      &j->d_win_bytes, &j->d_perm, &j->d_tiles, &j->d_queue, &j->d_scratch,
+                      &j->d_win_bytes, &j->d_perm, &j->d_tiles, &j->d_tile_scratch,
+                      &j->d_queue, &j->d_scratch,
                       &j->d_idx_rest, &j->d_idx_all, &j->d_idx_bytes, &j->d_off_rest,
                       &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops,
                       &j->d_rec_len, &j->d_rec_off, &j->d_rec_info, &j->d_rec_tmp,
@@ -585,6 +598,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     std::vector<uint32_t> tiles;   // uint4 per tile
     j->classes.clear();
     size_t scratch_words = 0;
+    std::vector<uint64_t> tile_off;   // word offset of each tile's band
     // band classes run side by side on 4 streams unless the tuning says otherwise
     // (1 = one after another on the context's stream)
     const int want_streams = ctx->tuning.class_streams == 0 ? 4 : ctx->tuning.class_streams;
@@ -631,13 +645,10 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             if (tuned > 0) per_cu = std::min(per_cu, tuned);
             int max_blocks = std::max(1, ctx->num_cus * per_cu);
             bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
-            const size_t sw = (size_t)bc.blocks * 4 * band_scratch_words(bc);
-            if (concurrent) {          // classes side by side: a scratch region each
-                bc.scratch_off = scratch_words;
-                scratch_words += sw;
-            } else {
-                bc.scratch_off = 0;
-                scratch_words = std::max(scratch_words, sw);
+            // every tile keeps its band until trace_kernel has walked it
+            for (uint32_t u = 0; u < bc.ntiles; ++u) {
+                tile_off.push_back(scratch_words);
+                scratch_words += tile_scratch_words(bc.nw, (int)tiles[4 * (bc.tile_begin + u) + 2]);
             }
             j->classes.push_back(bc);
             i = e;
@@ -671,6 +682,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     if ((rc = upload(ctx, j->d_win_bytes, win_bytes, win_bytes ? n_win_bytes : 0))) return rc;
     if ((rc = upload(ctx, j->d_perm, band_idx.data(), band_idx.size()))) return rc;
     if ((rc = upload(ctx, j->d_tiles, tiles.data(), tiles.size()))) return rc;
+    if ((rc = upload(ctx, j->d_tile_scratch, tile_off.data(), tile_off.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_rest, rest_idx.data(), rest_idx.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_all, all_idx.data(), all_idx.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_bytes, bytes_idx.data(), bytes_idx.size()))) return rc;

@@ -109,18 +109,28 @@ void srm_server::prepare(Request &r)
         char *dst = &r.reads_rev[r.off[i]];
         for (int j = 0; j < r.len[i]; ++j) dst[j] = comp(src[r.len[i] - 1 - j]);
     }
-    // both strands through the reference filter semantics
-    for (int s = 0; s < 2; ++s) {
-        srm_filter *F = nullptr;
-        int rc = dfilter ? srm_dfilter_run(dfilter, nullptr, r.reads.data(), r.off.data(),
-                                           r.len.data(), r.k.data(), n, s ? 4 : 0,
-                                           cfg.chunk_value, &F)
-                         : srm_filter_run(index, genome.data(), genome.size(), nullptr,
-                                          r.reads.data(), r.off.data(), r.len.data(), r.k.data(),
-                                          n, s ? 4 : 0, cfg.chunk_value, cfg.filter_threads, &F);
+    // both strands through the reference filter semantics (the device filter runs
+    // both over one upload of the reads)
+    srm_filter *both[2] = {nullptr, nullptr};
+    if (dfilter) {
+        int rc = srm_dfilter_run2(dfilter, nullptr, r.reads.data(), r.off.data(), r.len.data(),
+                                  r.k.data(), n, cfg.chunk_value, &both[0], &both[1]);
         if (rc != SRM_OK) {
             r.st.status = rc;
             return;
+        }
+    }
+    for (int s = 0; s < 2; ++s) {
+        srm_filter *F = both[s];
+        if (!F) {
+            int rc = srm_filter_run(index, genome.data(), genome.size(), nullptr, r.reads.data(),
+                                    r.off.data(), r.len.data(), r.k.data(), n, s ? 4 : 0,
+                                    cfg.chunk_value, cfg.filter_threads, &F);
+            if (rc != SRM_OK) {
+                if (both[1]) srm_filter_destroy(both[1]);
+                r.st.status = rc;
+                return;
+            }
         }
         uint64_t nc, nb, np;
         srm_filter_sizes(F, &nc, &nb, &np);
@@ -156,13 +166,15 @@ void srm_server::prepare(Request &r)
     r.read_words.assign((2 * total + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
     srm_pack_codes(r.reads.data(), total, r.read_words.data(), SRM_CODE_PAD);
     srm_pack_codes(r.reads_rev.data(), total, r.read_words.data(), SRM_CODE_PAD + total);
-    // device footprint: inputs, results, transcripts, tiles, banded scratch bound
+    // device footprint: inputs, results, transcripts, tiles and the band scratch
+    // every tile keeps until its traceback (at most 64 diagonals per lane; a tile
+    // per 64 candidates plus one partial tile per band class and L mod 8)
     size_t maxL = 0;
     for (auto L : r.len) maxL = std::max<size_t>(maxL, L);
     const size_t nc = r.cand.size();
-    const size_t waves = std::min<size_t>((nc + 63) / 64, 256 * 16);
+    const size_t tiles = (nc + 63) / 64 + 64;
     r.dev_bytes = r.read_words.size() * 4 + nc * (sizeof(srm_cand) + sizeof(srm_result) + 4 * 4) +
-                  r.ops_total * 4 + waves * (maxL + 1) * 8 * 64 * 4 + (1u << 20);
+                  r.ops_total * 4 + tiles * 64 * ((maxL + 8) / 8) * 64 * 4 + (1u << 20);
     if (cfg.gpu_records)   // positions, ids, lengths, offsets, entries and the text
         r.dev_bytes += nc * (8 + 4 + 8 + 8 + 16 + 2 * maxL + 48);
     r.st.n_reads = (long long)n;

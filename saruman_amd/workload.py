@@ -158,8 +158,41 @@ def needed_qs(reads: Reads):
     return sorted({filter_params(L, k)[0] for L, k in pairs})
 
 
+class Pinned:
+    """Page-locked host buffer (srm_host_alloc) viewed as a uint8 array."""
+
+    def __init__(self, nbytes: int):
+        self.lib = lib()
+        p = self.lib.srm_host_alloc(max(int(nbytes), 1))
+        if not p:
+            raise MemoryError(f"srm_host_alloc({nbytes}) failed")
+        self.p = p
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(int(nbytes), 1)).from_address(p))
+
+    def free(self):
+        if self.p:
+            self.array = None
+            self.lib.srm_host_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def pinned_reads(reads: Reads):
+    """The reads with their bases in page-locked host memory (the Pinned buffer
+    must outlive the returned Reads)."""
+    buf = Pinned(len(reads.seq))
+    buf.array[:len(reads.seq)] = np.frombuffer(reads.seq, np.uint8)
+    view = buf.array[:len(reads.seq)]
+    return Reads(view, reads.off, reads.length, reads.true_pos, reads.strand, reads.k), buf
+
+
 def _reads_args(reads: Reads):
-    rs = np.frombuffer(reads.seq, np.uint8)
+    rs = reads.seq if isinstance(reads.seq, np.ndarray) else np.frombuffer(reads.seq, np.uint8)
     off = np.ascontiguousarray(reads.off, np.uint64)
     ln = np.ascontiguousarray(reads.length, np.uint16)
     kk = np.ascontiguousarray(reads.k, np.uint8)
@@ -216,6 +249,18 @@ class DeviceFilter:
                                        ptr(ln), ptr(kk), reads.n, do_reverse, chunk_value,
                                        C.byref(h)), "srm_dfilter_run")
         return _collect(self.lib, h, do_reverse, time.perf_counter() - t0)
+
+    def run_both(self, reads: Reads, chunk_value: int):
+        """Both strands over one upload of the reads: (forward, reverse)."""
+        import time
+        hf, hr = _vp(), _vp()
+        rs, off, ln, kk = _reads_args(reads)
+        t0 = time.perf_counter()
+        check(self.lib.srm_dfilter_run2(self.h, ptr(self.genome.blocked), ptr(rs), ptr(off),
+                                        ptr(ln), ptr(kk), reads.n, chunk_value, C.byref(hf),
+                                        C.byref(hr)), "srm_dfilter_run2")
+        secs = time.perf_counter() - t0
+        return _collect(self.lib, hf, 0, secs), _collect(self.lib, hr, 4, secs)
 
     def stats(self) -> dict:
         ms = C.c_float()
