@@ -240,6 +240,7 @@ void band_kernel(BandArgs a)
         int since_sat = 0;
 
         // one DP column; last selects the free trailing verticals of column L
+        uint32_t lastmis[WW];   // mismatch fields of column L
         auto column = [&](int c, bool last) {
             // window for column c (c >= 2): shift by one code, take the next from the stream
             if (c >= 2) {
@@ -258,6 +259,10 @@ void band_kernel(BandArgs a)
             for (int j = 0; j < WW; ++j) {
                 const uint32_t x = gw[j] ^ rrep;
                 mis[j] = x | (x >> 1);   // bit 2i set <=> window code i != read code
+            }
+            if (last) {
+#pragma unroll
+                for (int j = 0; j < WW; ++j) lastmis[j] = mis[j];
             }
             int ifree = BMAX;            // first band cell whose top move is free
             if (last) ifree = w - 2 * e + 1 - c - dlo;
@@ -338,7 +343,34 @@ void band_kernel(BandArgs a)
         for (int c = 1; c < Lmax; ++c) {
             if (c < L) column(c, false);
         }
-        if (active) column(L, true);
+        // tail of the tile's scratch, for the walk's first (vertical) run: columns
+        // L and L - 1 by diagonal (nibble j of word m is diagonal 8m + j), then the
+        // equal-character bits of column L
+        uint32_t *__restrict__ tail = scr + (size_t)((Lmax + 8) >> 3) * BMAX * 64 + lane;
+        auto store_column = [&](int c, int word0) {
+#pragma unroll
+            for (int m = 0; m < NW; ++m) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int i = 8 * m + j;
+                    const int mv = UNIT ? v[i] + 2 * c + i : v[i];
+                    x |= ((uint32_t)mv & 15u) << (4 * j);
+                }
+                tail[(word0 + m) * 64] = x;
+            }
+        };
+        if (active) {
+            store_column(L - 1, NW);
+            column(L, true);
+            store_column(L, 0);
+            uint32_t eqw[2] = {0u, 0u};
+#pragma unroll
+            for (int i = 0; i < BMAX; ++i)
+                if (!((lastmis[i >> 4] >> (2 * (i & 15))) & 1u)) eqw[i >> 5] |= 1u << (i & 31);
+            tail[(2 * NW) * 64] = eqw[0];
+            tail[(2 * NW + 1) * 64] = eqw[1];
+        }
 
         // ---- score (cuda-functions.cu:495-497); the walk runs in trace_kernel
         if (active) {
@@ -412,19 +444,76 @@ void trace_kernel(BandArgs a)
     const int cap = w + k;
     int row = w, col = L, n = 0;
     uint32_t opw = 0;
+    bool done = false;
+    auto emit = [&](int op) {
+        opw |= (uint32_t)op << ((n & 15) * 2);
+        ++n;
+        if ((n & 15) == 0) {
+            ops[(n >> 4) - 1] = opw;
+            opw = 0;
+        }
+        row -= (op != 2);
+        col -= (op != 1);
+    };
+    {
+        // the first run, up column L (its free tail rows), from the packed tail
+        const uint32_t *__restrict__ tail =
+            scr + (size_t)((td.z + 8) >> 3) * BMAX * 64 + lane;
+        uint32_t tl[NW], tp[NW];
+#pragma unroll
+        for (int m = 0; m < NW; ++m) {
+            tl[m] = tail[m * 64];
+            tp[m] = tail[(NW + m) * 64];
+        }
+        const uint32_t eq0 = tail[(2 * NW) * 64], eq1 = tail[(2 * NW + 1) * 64];
+        auto nib = [&](const uint32_t (&X)[NW], int i) -> int {
+            if (i < 0 || i >= BMAX) return 15;
+            uint32_t x = X[0];
+#pragma unroll
+            for (int m = 1; m < NW; ++m) x = (i >> 3) == m ? X[m] : x;
+            return (int)((x >> ((i & 7) * 4)) & 15u);
+        };
+        const int wd = L - 1;
+        while (col == L) {
+            if (stop_row0 && row == 0) {
+                done = true;
+                break;
+            }
+            const int i = row - L - dlo;
+            const int vv = nib(tl, i);
+            const int Dg = row >= 1 ? imin(nib(tp, i), K) : SRM_FOREIGN;
+            const int Lf = imin(nib(tp, i + 1), K);
+            const int Tp = row >= 1 ? imin(nib(tl, i - 1), K) : SRM_FOREIGN;
+            const bool eq = i >= 0 && i < BMAX && ((i < 32 ? eq0 >> i : eq1 >> (i - 32)) & 1u);
+            const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, imax(row - 1, 0), L, w, s, e, eq);
+            if (op == 3 || n >= cap) {
+                done = true;
+                break;
+            }
+            emit(op);
+        }
+    }
     // cached words of diagonals ci-1, ci, ci+1 in column block cq (Wm, W0, Wp) and
-    // of ci, ci+1 in block cq-1 (P0, Pp)
+    // in block cq-1 (Pm, P0, Pp); a diagonal walk into block cq-1 takes the P words
+    // and fetches the next block's, which it needs only 8 steps later
     int ci = -1, cq = -1;
-    uint32_t Wm = 0, W0 = 0, Wp = 0, P0 = 0, Pp = 0;
+    uint32_t Wm = 0, W0 = 0, Wp = 0, Pm = 0, P0 = 0, Pp = 0;
     CodeCache rcc, gcc;
-    for (;;) {
+    while (!done) {
         if (stop_row0 && row == 0) break;
         const int i = row - col - dlo;
         const int q = col >> 3;
         if (i != ci || q != cq) {
-            W0 = band_word_or_k<BMAX>(scr, i, q, lane);
-            Wm = band_word_or_k<BMAX>(scr, i - 1, q, lane);
-            Wp = band_word_or_k<BMAX>(scr, i + 1, q, lane);
+            if (i == ci && q == cq - 1) {
+                Wm = Pm;
+                W0 = P0;
+                Wp = Pp;
+            } else {
+                Wm = band_word_or_k<BMAX>(scr, i - 1, q, lane);
+                W0 = band_word_or_k<BMAX>(scr, i, q, lane);
+                Wp = band_word_or_k<BMAX>(scr, i + 1, q, lane);
+            }
+            Pm = band_word_or_k<BMAX>(scr, i - 1, q - 1, lane);
             P0 = band_word_or_k<BMAX>(scr, i, q - 1, lane);
             Pp = band_word_or_k<BMAX>(scr, i + 1, q - 1, lane);
             ci = i;
@@ -449,14 +538,7 @@ void trace_kernel(BandArgs a)
         const bool eq = gcc.at(a.win_words, (int64_t)cd.win_off, ht) == rcc.at(a.read_words, rbase, wd);
         const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
         if (op == 3 || n >= cap) break;
-        opw |= (uint32_t)op << ((n & 15) * 2);
-        ++n;
-        if ((n & 15) == 0) {
-            ops[(n >> 4) - 1] = opw;
-            opw = 0;
-        }
-        row -= (op != 2);
-        col -= (op != 1);
+        emit(op);
     }
     if (n & 15) ops[n >> 4] = opw;
     r.n_ops = (uint16_t)n;

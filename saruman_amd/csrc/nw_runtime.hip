@@ -184,8 +184,12 @@ inline int band_cells(const srm_cand &c)
     return hi + k + 1;
 }
 
-// words of one tile's band scratch: 8*NW diagonals x 8-column blocks x 64 lanes
-inline size_t tile_scratch_words(int nw, int maxL) { return (size_t)nw * 8 * ((maxL + 8) / 8) * 64; }
+// words of one tile's band scratch: 8*NW diagonals x 8-column blocks x 64 lanes,
+// then the tail (columns L - 1 and L packed by diagonal, equal-character bits)
+inline size_t tile_scratch_words(int nw, int maxL)
+{
+    return ((size_t)nw * 8 * ((maxL + 8) / 8) + 2 * (size_t)nw + 2) * 64;
+}
 
 void destroy_graph(srm_job *j)
 {
