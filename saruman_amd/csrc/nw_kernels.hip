@@ -413,12 +413,36 @@ struct CodeCache {
     }
 };
 
+// nibble-parallel helpers of the walk: 8 nibbles of a band word at once
+// per-nibble (x - 1) mod 16
+__device__ __forceinline__ uint32_t nib_dec(uint32_t x)
+{
+    return ((x | 0x88888888u) - 0x11111111u) ^ (~x & 0x88888888u);
+}
+// bit 4j set <=> nibble j of x is nonzero
+__device__ __forceinline__ uint32_t nib_nz(uint32_t x)
+{
+    x |= x >> 1;
+    x |= x >> 2;
+    return x & 0x11111111u;
+}
+// 8 two-bit fields (bits 2j..2j+1) moved to nibbles (bits 4j..4j+1)
+__device__ __forceinline__ uint32_t spread2(uint32_t f)
+{
+    f = (f | (f << 8)) & 0x00FF00FFu;
+    f = (f | (f << 4)) & 0x0F0F0F0Fu;
+    return (f | (f << 2)) & 0x33333333u;
+}
+
 // Traceback of every hit of a band class (cuda-functions.cu:504-534, App. A.3),
 // one wave per tile of band_kernel, reading the band the forward pass left in
 // the tile's scratch.  Few registers and many resident waves: the walk's reloads
 // (one block of 8 columns per reload on a diagonal walk) wait for memory while
-// other waves run.
-template <int NW>
+// other waves run.  The walk goes block by block: a run of diagonal steps inside
+// a block of 8 columns is found from the block's word at once, the other steps
+// (gaps, the free rows of column L, the start of the read) one cell at a time;
+// column 0 comes from its closed form max(0, h - 2s) (Q7) without loads.
+template <int NW, bool UNIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 void trace_kernel(BandArgs a)
 {
@@ -493,49 +517,121 @@ void trace_kernel(BandArgs a)
             emit(op);
         }
     }
-    // cached words of diagonals ci-1, ci, ci+1 in column block cq (Wm, W0, Wp) and
-    // in block cq-1 (Pm, P0, Pp); a diagonal walk into block cq-1 takes the P words
-    // and fetches the next block's, which it needs only 8 steps later
+    // the walk's block: words of diagonal ci in column blocks cq (W0) and cq-1
+    // (P0); the neighbour diagonals' words (Wm: ci-1, Wp and Pp: ci+1) are
+    // loaded only when a cell of the block needs them (a step that is not diagonal)
     int ci = -1, cq = -1;
-    uint32_t Wm = 0, W0 = 0, Wp = 0, Pm = 0, P0 = 0, Pp = 0;
-    CodeCache rcc, gcc;
+    bool side = false;
+    uint32_t W0 = 0, P0 = 0, Wm = 0, Wp = 0, Pp = 0;
+    uint32_t mis = 0;          // unit costs: bit 4j set <=> characters differ at column 8cq+j
+    CodeCache rcc, gcc;        // other costs: the characters one cell at a time
+    const int64_t woff = (int64_t)cd.win_off;
+    auto m0 = [&](int h, int i) -> int {   // stored nibble of column 0, row h, band cell i
+        return (i >= 0 && i < BMAX) ? imin(imax(h - 2 * s, 0), K) : 15;
+    };
     while (!done) {
         if (stop_row0 && row == 0) break;
+        if (col == 0) {
+            // column 0: M[h][0] = max(0, h - 2s) clipped at K, as the forward pass
+            // stored it; only a vertical step or the stop remain (Dg, Lf = COL0)
+            const int i = row - dlo;
+            const int vv = m0(row, i);
+            const int Tp = row >= 1 ? imin(m0(row - 1, i - 1), K) : SRM_FOREIGN;
+            const int ht = imax(row - 1, 0);
+            bool eq = false;
+            if (!UNIT && SRM_COL0 + ma == vv)
+                eq = gcc.at(a.win_words, woff, ht) == rcc.at(a.read_words, rbase, 0);
+            const int op = decide(ma, mm, vv, SRM_COL0, SRM_COL0, Tp, 0, ht, L, w, s, e, eq);
+            if (op == 3 || n >= cap) break;
+            emit(op);
+            continue;
+        }
         const int i = row - col - dlo;
         const int q = col >> 3;
         if (i != ci || q != cq) {
-            if (i == ci && q == cq - 1) {
-                Wm = Pm;
-                W0 = P0;
-                Wp = Pp;
-            } else {
-                Wm = band_word_or_k<BMAX>(scr, i - 1, q, lane);
-                W0 = band_word_or_k<BMAX>(scr, i, q, lane);
-                Wp = band_word_or_k<BMAX>(scr, i + 1, q, lane);
-            }
-            Pm = band_word_or_k<BMAX>(scr, i - 1, q - 1, lane);
+            W0 = (i == ci && q == cq - 1) ? P0 : band_word_or_k<BMAX>(scr, i, q, lane);
             P0 = band_word_or_k<BMAX>(scr, i, q - 1, lane);
-            Pp = band_word_or_k<BMAX>(scr, i + 1, q - 1, lane);
+            if (UNIT) {
+                // codes of the block's 8 cells: read column 8q+j-1, window row
+                // (i + dlo + 8q + j) - 1; a run never uses rows or columns below 1
+                const uint32_t rc = fetch16(a.read_words, rbase + 8 * q - 1);
+                const uint32_t gc = fetch16(a.win_words, woff + (i + dlo + 8 * q) - 1);
+                const uint32_t x = rc ^ gc;
+                mis = spread2((x | (x >> 1)) & 0x5555u);
+            }
             ci = i;
             cq = q;
+            side = false;
         }
-        const uint32_t sh = (uint32_t)(col & 7) * 4u;
+        const int j0 = col & 7;
+        if (UNIT) {
+            // unit costs: cell j steps diagonally iff v_j == D_j + 1 (mismatch cost)
+            // or v_j == D_j with equal characters, D_j the diagonal predecessor's
+            // stored nibble (nibble j-1, or P0's nibble 7 for j = 0).  On the walk
+            // v <= k < K, and a successful step keeps the next cell <= k, so the
+            // clip of D at K never decides a cell of the run.
+            const uint32_t N = __builtin_amdgcn_alignbit(W0, P0, 28);
+            const uint32_t fail_mm = nib_nz(nib_dec(W0) ^ N) | (~nib_nz(W0) & 0x11111111u);
+            const uint32_t fail_ma = nib_nz(W0 ^ N) | mis;
+            const uint32_t fail = fail_mm & fail_ma & ((0x2u << (4 * j0)) - 1u);
+            int rr = fail ? j0 - ((31 - __builtin_clz(fail)) >> 2) : j0 + 1;
+            rr = imin(imin(rr, col), imin(row, cap - n));
+            if (rr > 0) {
+                if (((n & 15) + rr) >= 16) {   // the run completes the current ops word
+                    ops[n >> 4] = opw;
+                    opw = 0;
+                }
+                n += rr;
+                row -= rr;
+                col -= rr;
+                continue;
+            }
+        } else {
+            // other costs: the first two rules of decide() one cell at a time
+            const int rmax = imin(j0 + 1, imin(imin(col, row), cap - n));
+            int rr = 0;
+            while (rr < rmax) {
+                const int c2 = col - rr, j2 = c2 & 7;
+                const int v2 = (int)((W0 >> (4 * j2)) & 15u);
+                const int d2 = imin(j2 ? (int)((W0 >> (4 * j2 - 4)) & 15u) : (int)(P0 >> 28), K);
+                if (d2 + mm != v2) {
+                    if (d2 + ma != v2) break;
+                    if (gcc.at(a.win_words, woff, row - rr - 1) != rcc.at(a.read_words, rbase, c2 - 1))
+                        break;
+                }
+                ++rr;
+            }
+            if (rr > 0) {
+                if (((n & 15) + rr) >= 16) {
+                    ops[n >> 4] = opw;
+                    opw = 0;
+                }
+                n += rr;
+                row -= rr;
+                col -= rr;
+                continue;
+            }
+        }
+        // one cell that is not on a diagonal run (col >= 1)
+        if (!side) {
+            Wm = band_word_or_k<BMAX>(scr, i - 1, q, lane);
+            Wp = band_word_or_k<BMAX>(scr, i + 1, q, lane);
+            Pp = band_word_or_k<BMAX>(scr, i + 1, q - 1, lane);
+            side = true;
+        }
+        const uint32_t sh = (uint32_t)j0 * 4u;
         // a nibble of 15 outside the band reads as K (never equal to v <= k)
         const int vv = (int)((W0 >> sh) & 15u);
-        int Dg, Lf;
-        if (col != 0) {
-            const bool in = (col & 7) != 0;
-            const uint32_t dgw = in ? W0 : P0, lfw = in ? Wp : Pp;
-            const uint32_t psh = in ? sh - 4u : 28u;
-            Dg = row >= 1 ? imin((int)((dgw >> psh) & 15u), K) : SRM_FOREIGN;
-            Lf = imin((int)((lfw >> psh) & 15u), K);
-        } else {
-            Dg = SRM_COL0;
-            Lf = SRM_COL0;
-        }
+        const bool in = j0 != 0;
+        const uint32_t dgw = in ? W0 : P0, lfw = in ? Wp : Pp;
+        const uint32_t psh = in ? sh - 4u : 28u;
+        const int Dg = row >= 1 ? imin((int)((dgw >> psh) & 15u), K) : SRM_FOREIGN;
+        const int Lf = imin((int)((lfw >> psh) & 15u), K);
         const int Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
-        const int wd = imax(col - 1, 0), ht = imax(row - 1, 0);
-        const bool eq = gcc.at(a.win_words, (int64_t)cd.win_off, ht) == rcc.at(a.read_words, rbase, wd);
+        const int wd = col - 1, ht = imax(row - 1, 0);
+        // unit costs: the equal-character bit matters only with Dg = v, i.e. row >= 1
+        const bool eq = UNIT ? ((mis >> sh) & 1u) == 0u
+                             : gcc.at(a.win_words, woff, ht) == rcc.at(a.read_words, rbase, wd);
         const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
         if (op == 3 || n >= cap) break;
         emit(op);
@@ -640,26 +736,29 @@ hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStre
 }
 
 template <int NW>
-static hipError_t launch_trace_nw(const BandArgs &a, hipStream_t st)
+static hipError_t launch_trace_nw(const BandArgs &a, bool unit, hipStream_t st)
 {
     dim3 block(256), grid((a.ntiles + 3) / 4);
-    hipLaunchKernelGGL((trace_kernel<NW>), grid, block, 0, st, a);
+    if (unit)
+        hipLaunchKernelGGL((trace_kernel<NW, true>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((trace_kernel<NW, false>), grid, block, 0, st, a);
     return hipGetLastError();
 }
 
-// one wave per tile of the class
-hipError_t launch_trace(const BandArgs &a, int nw, hipStream_t st)
+// one wave per tile of the class; unit: match 0, mismatch 1
+hipError_t launch_trace(const BandArgs &a, int nw, bool unit, hipStream_t st)
 {
     if (a.ntiles == 0) return hipSuccess;
     switch (nw) {
-    case 1: return launch_trace_nw<1>(a, st);
-    case 2: return launch_trace_nw<2>(a, st);
-    case 3: return launch_trace_nw<3>(a, st);
-    case 4: return launch_trace_nw<4>(a, st);
-    case 5: return launch_trace_nw<5>(a, st);
-    case 6: return launch_trace_nw<6>(a, st);
-    case 7: return launch_trace_nw<7>(a, st);
-    case 8: return launch_trace_nw<8>(a, st);
+    case 1: return launch_trace_nw<1>(a, unit, st);
+    case 2: return launch_trace_nw<2>(a, unit, st);
+    case 3: return launch_trace_nw<3>(a, unit, st);
+    case 4: return launch_trace_nw<4>(a, unit, st);
+    case 5: return launch_trace_nw<5>(a, unit, st);
+    case 6: return launch_trace_nw<6>(a, unit, st);
+    case 7: return launch_trace_nw<7>(a, unit, st);
+    case 8: return launch_trace_nw<8>(a, unit, st);
     default: return hipErrorInvalidValue;
     }
 }
