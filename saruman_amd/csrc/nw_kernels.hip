@@ -184,7 +184,7 @@ __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, 
 // budget is 512 / waves VGPRs
 template <int NW>
 struct BandWaves {
-    static constexpr int value = NW <= 1 ? 8 : NW <= 2 ? 5 : NW <= 3 ? 4 : NW <= 5 ? 3 : 2;
+    static constexpr int value = NW <= 1 ? 8 : NW <= 2 ? 6 : NW <= 3 ? 5 : NW <= 4 ? 4 : NW <= 6 ? 3 : 2;
 };
 
 template <int NW, bool UNIT>
@@ -201,7 +201,14 @@ void band_kernel(BandArgs a)
         if (lane == 0) tile = atomicAdd(a.queue, 1u);
         tile = __builtin_amdgcn_readfirstlane(tile);
         if (tile >= a.ntiles) break;
-        const uint4 td = a.tiles[tile];     // x: perm start, y: count, z: max L, w: max k
+        // x: perm start, y: count, z: max L, w: max k -- one tile per wave: made
+        // wave-uniform so the column loop and its every-8/16-column steps stay
+        // scalar branches
+        uint4 td = a.tiles[tile];
+        td.x = __builtin_amdgcn_readfirstlane(td.x);
+        td.y = __builtin_amdgcn_readfirstlane(td.y);
+        td.z = __builtin_amdgcn_readfirstlane(td.z);
+        td.w = __builtin_amdgcn_readfirstlane(td.w);
         uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
         const bool active = (uint32_t)lane < td.y;
         const uint32_t t = active ? a.perm[td.x + lane] : 0u;
@@ -241,7 +248,7 @@ void band_kernel(BandArgs a)
 
         // one DP column; last selects the free trailing verticals of column L
         uint32_t lastmis[WW];   // mismatch fields of column L
-        auto column = [&](int c, bool last) {
+        auto column = [&](int c, bool last, bool satc) {
             // window for column c (c >= 2): shift by one code, take the next from the stream
             if (c >= 2) {
                 if (((c - 2) & 15) == 0) gnext = fetch16(a.win_words, gbase + 16 * WW + (c - 2));
@@ -253,7 +260,8 @@ void band_kernel(BandArgs a)
             if (((c - 1) & 15) == 0) rbuf = fetch16(a.read_words, rbase + (c - 1));
             const uint32_t rc = rbuf & 3u;
             rbuf >>= 2;
-            const uint32_t rrep = rc * 0x55555555u;
+            const uint32_t r8 = rc * 0x5555u;          // 24-bit multiply (full rate)
+            const uint32_t rrep = r8 | (r8 << 16);     // the read code in all 16 fields
             uint32_t mis[WW];
 #pragma unroll
             for (int j = 0; j < WW; ++j) {
@@ -266,8 +274,7 @@ void band_kernel(BandArgs a)
             }
             int ifree = BMAX;            // first band cell whose top move is free
             if (last) ifree = w - 2 * e + 1 - c - dlo;
-            const bool sat = !UNIT || last || ++since_sat >= satp;
-            if (sat) since_sat = 0;
+            const bool sat = !UNIT || last || satc;
             const uint32_t sh = (uint32_t)(c & 7) * 4u;
             if (UNIT) {
                 uint32_t misf[WW];
@@ -312,16 +319,14 @@ void band_kernel(BandArgs a)
                     for (int i = 0; i < BMAX; ++i) v[i] = (i == i0) ? imin(c, K) : v[i];
                 }
             }
-            // nibble c & 7 of every diagonal's word; a block is stored when complete
-            if ((c & 7) == 0) {
+            // nibble c & 7 of every diagonal's word (an add: unit-cost values are
+            // potentials, which may be negative; the block's sum is exact mod 2^32);
+            // a block is stored when complete and its words start again from 0
 #pragma unroll
-                for (int i = 0; i < BMAX; ++i) acc[i] = (uint32_t)v[i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < BMAX; ++i)
-                    acc[i] = UNIT ? ((uint32_t)v[i] << sh) + acc[i] : ((uint32_t)v[i] << sh) | acc[i];
-            }
+            for (int i = 0; i < BMAX; ++i) acc[i] += (uint32_t)v[i] << sh;
             if ((c & 7) == 7 || last) {
+                // streaming stores: the band is read back only by the walk, much
+                // later; the code words the next columns fetch stay in L2
                 uint32_t *__restrict__ blk = scr + (size_t)(c >> 3) * BMAX * 64 + lane;
                 if (UNIT) {
                     // back from W: nibble j of word (q, i) gets 2(8q + j) + i, i.e. the
@@ -332,16 +337,25 @@ void band_kernel(BandArgs a)
                     const uint32_t base = 0xECA86420u & msk;
                     const uint32_t q16 = (uint32_t)(c >> 3) * 16u;
 #pragma unroll
-                    for (int i = 0; i < BMAX; ++i) blk[i * 64] = acc[i] + (q16 + (uint32_t)i) * ones + base;
+                    for (int i = 0; i < BMAX; ++i)
+                        __builtin_nontemporal_store(acc[i] + (q16 + (uint32_t)i) * ones + base,
+                                                    &blk[i * 64]);
                 } else {
 #pragma unroll
-                    for (int i = 0; i < BMAX; ++i) blk[i * 64] = acc[i];
+                    for (int i = 0; i < BMAX; ++i) __builtin_nontemporal_store(acc[i], &blk[i * 64]);
                 }
+#pragma unroll
+                for (int i = 0; i < BMAX; ++i) acc[i] = 0u;
             }
         };
 
-        for (int c = 1; c < Lmax; ++c) {
-            if (c < L) column(c, false);
+        for (int c0 = 1; c0 < Lmax; ++c0) {
+            // the column is the same on every lane: kept scalar, so the steps taken
+            // every 8 and 16 columns are scalar branches, not masked ones
+            const int c = __builtin_amdgcn_readfirstlane(c0);
+            const bool satc = ++since_sat >= satp;   // every satp columns, on every lane
+            if (satc) since_sat = 0;
+            if (c < L) column(c, false, satc);
         }
         // tail of the tile's scratch, for the walk's first (vertical) run: columns
         // L and L - 1 by diagonal (nibble j of word m is diagonal 8m + j), then the
@@ -357,12 +371,12 @@ void band_kernel(BandArgs a)
                     const int mv = UNIT ? v[i] + 2 * c + i : v[i];
                     x |= ((uint32_t)mv & 15u) << (4 * j);
                 }
-                tail[(word0 + m) * 64] = x;
+                __builtin_nontemporal_store(x, &tail[(word0 + m) * 64]);
             }
         };
         if (active) {
             store_column(L - 1, NW);
-            column(L, true);
+            column(L, true, true);
             store_column(L, 0);
             uint32_t eqw[2] = {0u, 0u};
 #pragma unroll
