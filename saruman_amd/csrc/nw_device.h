@@ -44,6 +44,8 @@ struct BandArgs {
     const uint64_t *tile_scratch;   // [ntiles] word offset of each tile's band in scratch
     srm_result *res;
     uint32_t *ops;
+    int code_words;                 // trace_kernel: LDS words per lane staging the read's
+                                    // codes (0: the codes come from read_words)
 };
 
 struct RecArgs {
@@ -74,7 +76,7 @@ hipError_t launch_rec_write(const RecArgs &a, hipStream_t st);
 hipError_t rec_scan(void *tmp, size_t &tmp_bytes, const uint64_t *len, uint64_t *off, int n,
                     hipStream_t st);
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
-hipError_t launch_trace(const BandArgs &a, int nw, bool unit, hipStream_t st);
+hipError_t launch_trace(const BandArgs &a, int nw, bool unit, int maxL, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);

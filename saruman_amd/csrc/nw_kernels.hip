@@ -184,7 +184,8 @@ __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, 
 // budget is 512 / waves VGPRs
 template <int NW>
 struct BandWaves {
-    static constexpr int value = NW <= 1 ? 8 : NW <= 2 ? 6 : NW <= 3 ? 5 : NW <= 4 ? 4 : NW <= 6 ? 3 : 2;
+    static constexpr int value =
+        NW <= 1 ? 8 : NW <= 2 ? 6 : NW <= 3 ? 5 : NW <= 4 ? 4 : NW <= 6 ? 3 : 2;
 };
 
 template <int NW, bool UNIT>
@@ -253,7 +254,8 @@ void band_kernel(BandArgs a)
             if (c >= 2) {
                 if (((c - 2) & 15) == 0) gnext = fetch16(a.win_words, gbase + 16 * WW + (c - 2));
 #pragma unroll
-                for (int j = 0; j < WW - 1; ++j) gw[j] = __builtin_amdgcn_alignbit(gw[j + 1], gw[j], 2);
+                for (int j = 0; j < WW - 1; ++j)
+                    gw[j] = __builtin_amdgcn_alignbit(gw[j + 1], gw[j], 2);
                 gw[WW - 1] = __builtin_amdgcn_alignbit(gnext, gw[WW - 1], 2);
                 gnext >>= 2;
             }
@@ -342,7 +344,8 @@ void band_kernel(BandArgs a)
                                                     &blk[i * 64]);
                 } else {
 #pragma unroll
-                    for (int i = 0; i < BMAX; ++i) __builtin_nontemporal_store(acc[i], &blk[i * 64]);
+                    for (int i = 0; i < BMAX; ++i)
+                        __builtin_nontemporal_store(acc[i], &blk[i * 64]);
                 }
 #pragma unroll
                 for (int i = 0; i < BMAX; ++i) acc[i] = 0u;
@@ -411,21 +414,6 @@ __device__ __forceinline__ uint32_t band_word_or_k(const uint32_t *__restrict__ 
 {
     return (i >= 0 && i < BMAX && q >= 0) ? band_word<BMAX>(scr, i, q, lane) : 0xFFFFFFFFu;
 }
-
-// 16 codes of a sequence starting at a multiple of 16 at or below idx
-struct CodeCache {
-    int64_t base = INT64_MIN;
-    uint32_t word = 0;
-    __device__ __forceinline__ int at(const uint32_t *__restrict__ words, int64_t org, int idx)
-    {
-        const int64_t b = (int64_t)(idx & ~15);
-        if (b != base) {
-            base = b;
-            word = fetch16(words, org + b);
-        }
-        return (int)((word >> ((idx & 15) * 2)) & 3u);
-    }
-};
 
 // nibble-parallel helpers of the walk: 8 nibbles of a band word at once
 // per-nibble (x - 1) mod 16
@@ -497,19 +485,19 @@ void trace_kernel(BandArgs a)
         // the first run, up column L (its free tail rows), from the packed tail
         const uint32_t *__restrict__ tail =
             scr + (size_t)((td.z + 8) >> 3) * BMAX * 64 + lane;
-        uint32_t tl[NW], tp[NW];
+        // vectors in registers: a lane's word pick (i >> 3) becomes a select
+        // chain, never a private array in scratch memory
+        typedef uint32_t TailWords __attribute__((ext_vector_type(NW)));
+        TailWords tl, tp;
 #pragma unroll
         for (int m = 0; m < NW; ++m) {
             tl[m] = tail[m * 64];
             tp[m] = tail[(NW + m) * 64];
         }
         const uint32_t eq0 = tail[(2 * NW) * 64], eq1 = tail[(2 * NW + 1) * 64];
-        auto nib = [&](const uint32_t (&X)[NW], int i) -> int {
+        auto nib = [&](const TailWords &X, int i) -> int {
             if (i < 0 || i >= BMAX) return 15;
-            uint32_t x = X[0];
-#pragma unroll
-            for (int m = 1; m < NW; ++m) x = (i >> 3) == m ? X[m] : x;
-            return (int)((x >> ((i & 7) * 4)) & 15u);
+            return (int)((X[i >> 3] >> ((i & 7) * 4)) & 15u);
         };
         const int wd = L - 1;
         while (col == L) {
@@ -531,32 +519,48 @@ void trace_kernel(BandArgs a)
             emit(op);
         }
     }
+    // the read's codes, staged in this wave's LDS (lane-interleaved words: the
+    // lanes' reads of one word index hit different banks); word u holds the
+    // read_words word ((rbase - 1) >> 4) + u
+    extern __shared__ uint32_t trace_lds[];
+    const int rwn = a.code_words;
+    uint32_t *__restrict__ lcodes = trace_lds + (size_t)(threadIdx.x >> 6) * 64 * rwn + lane;
+    if (rwn && !done) {
+        const uint32_t *__restrict__ src = a.read_words + ((rbase - 1) >> 4);
+        const int nwd = ((L + 15) >> 4) + 3;   // <= rwn: L <= the class's max L
+        for (int u = 0; u < nwd; ++u) lcodes[u * 64] = src[u];
+    }
     // the walk's block: words of diagonal ci in column blocks cq (W0) and cq-1
-    // (P0); the neighbour diagonals' words (Wm: ci-1, Wp and Pp: ci+1) are
-    // loaded only when a cell of the block needs them (a step that is not diagonal)
+    // (P0) and the character comparison of its 8 cells; the neighbour diagonals'
+    // words (Wm: ci-1, Wp and Pp: ci+1) are loaded only when a cell of the block
+    // needs them (a step that is not diagonal)
     int ci = -1, cq = -1;
     bool side = false;
     uint32_t W0 = 0, P0 = 0, Wm = 0, Wp = 0, Pp = 0;
-    uint32_t mis = 0;          // unit costs: bit 4j set <=> characters differ at column 8cq+j
-    CodeCache rcc, gcc;        // other costs: the characters one cell at a time
+    uint32_t mis = 0;          // bit 4j set <=> the characters of cell 8cq+j differ
     const int64_t woff = (int64_t)cd.win_off;
     auto m0 = [&](int h, int i) -> int {   // stored nibble of column 0, row h, band cell i
         return (i >= 0 && i < BMAX) ? imin(imax(h - 2 * s, 0), K) : 15;
     };
     while (!done) {
-        if (stop_row0 && row == 0) break;
+        if (stop_row0 && row == 0) {
+            done = true;
+            break;
+        }
         if (col == 0) {
             // column 0: M[h][0] = max(0, h - 2s) clipped at K, as the forward pass
-            // stored it; only a vertical step or the stop remain (Dg, Lf = COL0)
+            // stored it; a vertical step or the stop remain (Dg = Lf = COL0;
+            // with match >= 0, as on every banded candidate, the characters
+            // never decide)
             const int i = row - dlo;
             const int vv = m0(row, i);
             const int Tp = row >= 1 ? imin(m0(row - 1, i - 1), K) : SRM_FOREIGN;
-            const int ht = imax(row - 1, 0);
-            bool eq = false;
-            if (!UNIT && SRM_COL0 + ma == vv)
-                eq = gcc.at(a.win_words, woff, ht) == rcc.at(a.read_words, rbase, 0);
-            const int op = decide(ma, mm, vv, SRM_COL0, SRM_COL0, Tp, 0, ht, L, w, s, e, eq);
-            if (op == 3 || n >= cap) break;
+            const int op = decide(ma, mm, vv, SRM_COL0, SRM_COL0, Tp, 0, imax(row - 1, 0), L,
+                                  w, s, e, false);
+            if (op == 3 || n >= cap) {
+                done = true;
+                break;
+            }
             emit(op);
             continue;
         }
@@ -565,19 +569,28 @@ void trace_kernel(BandArgs a)
         if (i != ci || q != cq) {
             W0 = (i == ci && q == cq - 1) ? P0 : band_word_or_k<BMAX>(scr, i, q, lane);
             P0 = band_word_or_k<BMAX>(scr, i, q - 1, lane);
-            if (UNIT) {
-                // codes of the block's 8 cells: read column 8q+j-1, window row
-                // (i + dlo + 8q + j) - 1; a run never uses rows or columns below 1
-                const uint32_t rc = fetch16(a.read_words, rbase + 8 * q - 1);
-                const uint32_t gc = fetch16(a.win_words, woff + (i + dlo + 8 * q) - 1);
-                const uint32_t x = rc ^ gc;
-                mis = spread2((x | (x >> 1)) & 0x5555u);
+            // codes of the block's 8 cells: read column 8q+j-1, window row
+            // (i + dlo + 8q + j) - 1; exact for rows and columns >= 1, the only
+            // cells whose characters can decide (Dg is FOREIGN on row 0)
+            uint32_t rc;
+            if (rwn) {   // words wi, wi + 1 < ((L - 1) >> 4) + 3 <= nwd (8q <= L - 1)
+                // code rbase + 8q - 1 sits at place o of the staged codes
+                const int o = (int)((rbase - 1) & 15) + 8 * q;
+                const int wi = o >> 4;
+                rc = __builtin_amdgcn_alignbit(lcodes[(wi + 1) * 64], lcodes[wi * 64],
+                                               (uint32_t)(o & 15) * 2u);
+            } else {
+                rc = fetch16(a.read_words, rbase + 8 * q - 1);
             }
+            const uint32_t gc = fetch16(a.win_words, woff + (i + dlo + 8 * q) - 1);
+            const uint32_t x = rc ^ gc;
+            mis = spread2((x | (x >> 1)) & 0x5555u);
             ci = i;
             cq = q;
             side = false;
         }
         const int j0 = col & 7;
+        int rr;
         if (UNIT) {
             // unit costs: cell j steps diagonally iff v_j == D_j + 1 (mismatch cost)
             // or v_j == D_j with equal characters, D_j the diagonal predecessor's
@@ -588,43 +601,31 @@ void trace_kernel(BandArgs a)
             const uint32_t fail_mm = nib_nz(nib_dec(W0) ^ N) | (~nib_nz(W0) & 0x11111111u);
             const uint32_t fail_ma = nib_nz(W0 ^ N) | mis;
             const uint32_t fail = fail_mm & fail_ma & ((0x2u << (4 * j0)) - 1u);
-            int rr = fail ? j0 - ((31 - __builtin_clz(fail)) >> 2) : j0 + 1;
+            rr = fail ? j0 - ((31 - __builtin_clz(fail)) >> 2) : j0 + 1;
             rr = imin(imin(rr, col), imin(row, cap - n));
-            if (rr > 0) {
-                if (((n & 15) + rr) >= 16) {   // the run completes the current ops word
-                    ops[n >> 4] = opw;
-                    opw = 0;
-                }
-                n += rr;
-                row -= rr;
-                col -= rr;
-                continue;
-            }
         } else {
             // other costs: the first two rules of decide() one cell at a time
             const int rmax = imin(j0 + 1, imin(imin(col, row), cap - n));
-            int rr = 0;
+            rr = 0;
             while (rr < rmax) {
-                const int c2 = col - rr, j2 = c2 & 7;
+                const int j2 = j0 - rr;
                 const int v2 = (int)((W0 >> (4 * j2)) & 15u);
-                const int d2 = imin(j2 ? (int)((W0 >> (4 * j2 - 4)) & 15u) : (int)(P0 >> 28), K);
-                if (d2 + mm != v2) {
-                    if (d2 + ma != v2) break;
-                    if (gcc.at(a.win_words, woff, row - rr - 1) != rcc.at(a.read_words, rbase, c2 - 1))
-                        break;
-                }
+                const int d2 =
+                    imin(j2 ? (int)((W0 >> (4 * j2 - 4)) & 15u) : (int)(P0 >> 28), K);
+                if (d2 + mm != v2 && (d2 + ma != v2 || ((mis >> (4 * j2)) & 1u))) break;
                 ++rr;
             }
-            if (rr > 0) {
-                if (((n & 15) + rr) >= 16) {
-                    ops[n >> 4] = opw;
-                    opw = 0;
-                }
-                n += rr;
-                row -= rr;
-                col -= rr;
-                continue;
+        }
+        if (rr > 0) {
+            // rr diagonal steps: 0 bits in the transcript
+            if (((n & 15) + rr) >= 16) {   // the run completes the current ops word
+                ops[n >> 4] = opw;
+                opw = 0;
             }
+            n += rr;
+            row -= rr;
+            col -= rr;
+            continue;
         }
         // one cell that is not on a diagonal run (col >= 1)
         if (!side) {
@@ -642,12 +643,13 @@ void trace_kernel(BandArgs a)
         const int Dg = row >= 1 ? imin((int)((dgw >> psh) & 15u), K) : SRM_FOREIGN;
         const int Lf = imin((int)((lfw >> psh) & 15u), K);
         const int Tp = row >= 1 ? imin((int)((Wm >> sh) & 15u), K) : SRM_FOREIGN;
-        const int wd = col - 1, ht = imax(row - 1, 0);
-        // unit costs: the equal-character bit matters only with Dg = v, i.e. row >= 1
-        const bool eq = UNIT ? ((mis >> sh) & 1u) == 0u
-                             : gcc.at(a.win_words, woff, ht) == rcc.at(a.read_words, rbase, wd);
-        const int op = decide(ma, mm, vv, Dg, Lf, Tp, wd, ht, L, w, s, e, eq);
-        if (op == 3 || n >= cap) break;
+        const bool eq = ((mis >> sh) & 1u) == 0u;   // decides only with row >= 1
+        const int op =
+            decide(ma, mm, vv, Dg, Lf, Tp, col - 1, imax(row - 1, 0), L, w, s, e, eq);
+        if (op == 3 || n >= cap) {
+            done = true;
+            break;
+        }
         emit(op);
     }
     if (n & 15) ops[n >> 4] = opw;
@@ -750,29 +752,38 @@ hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStre
 }
 
 template <int NW>
-static hipError_t launch_trace_nw(const BandArgs &a, bool unit, hipStream_t st)
+static hipError_t launch_trace_nw(const BandArgs &a, bool unit, size_t lds, hipStream_t st)
 {
     dim3 block(256), grid((a.ntiles + 3) / 4);
     if (unit)
-        hipLaunchKernelGGL((trace_kernel<NW, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((trace_kernel<NW, true>), grid, block, lds, st, a);
     else
-        hipLaunchKernelGGL((trace_kernel<NW, false>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((trace_kernel<NW, false>), grid, block, lds, st, a);
     return hipGetLastError();
 }
 
-// one wave per tile of the class; unit: match 0, mismatch 1
-hipError_t launch_trace(const BandArgs &a, int nw, bool unit, hipStream_t st)
+// words per lane that stage a read of length <= maxL in LDS: codes from one
+// before the read's first to 16 past its last, from word boundaries
+static int trace_code_words(int maxL) { return ((maxL + 15) >> 4) + 3; }
+
+// one wave per tile of the class; unit: match 0, mismatch 1.  Reads of up to
+// 600 codes are staged in LDS (at most 40 KiB per block of 4 waves); longer
+// ones are read from read_words.
+hipError_t launch_trace(const BandArgs &args, int nw, bool unit, int maxL, hipStream_t st)
 {
-    if (a.ntiles == 0) return hipSuccess;
+    if (args.ntiles == 0) return hipSuccess;
+    BandArgs a = args;
+    a.code_words = maxL <= 600 ? trace_code_words(maxL) : 0;
+    const size_t lds = (size_t)a.code_words * 256 * 4;
     switch (nw) {
-    case 1: return launch_trace_nw<1>(a, unit, st);
-    case 2: return launch_trace_nw<2>(a, unit, st);
-    case 3: return launch_trace_nw<3>(a, unit, st);
-    case 4: return launch_trace_nw<4>(a, unit, st);
-    case 5: return launch_trace_nw<5>(a, unit, st);
-    case 6: return launch_trace_nw<6>(a, unit, st);
-    case 7: return launch_trace_nw<7>(a, unit, st);
-    case 8: return launch_trace_nw<8>(a, unit, st);
+    case 1: return launch_trace_nw<1>(a, unit, lds, st);
+    case 2: return launch_trace_nw<2>(a, unit, lds, st);
+    case 3: return launch_trace_nw<3>(a, unit, lds, st);
+    case 4: return launch_trace_nw<4>(a, unit, lds, st);
+    case 5: return launch_trace_nw<5>(a, unit, lds, st);
+    case 6: return launch_trace_nw<6>(a, unit, lds, st);
+    case 7: return launch_trace_nw<7>(a, unit, lds, st);
+    case 8: return launch_trace_nw<8>(a, unit, lds, st);
     default: return hipErrorInvalidValue;
     }
 }
