@@ -185,7 +185,7 @@ __device__ __forceinline__ uint32_t band_word(const uint32_t *__restrict__ scr, 
 template <int NW>
 struct BandWaves {
     static constexpr int value =
-        NW <= 1 ? 8 : NW <= 2 ? 6 : NW <= 3 ? 5 : NW <= 4 ? 4 : NW <= 6 ? 3 : 2;
+        NW <= 1 ? 8 : NW <= 2 ? 7 : NW <= 3 ? 6 : NW <= 5 ? 4 : NW <= 7 ? 3 : 2;
 };
 
 template <int NW, bool UNIT>
@@ -298,8 +298,13 @@ void band_kernel(BandArgs a)
                     up = nv;
                 }
                 if (sat && !last) {
+                    // the periodic clip uses the tile's bound (max k + 1 >= K, the same
+                    // on every lane: the bounds are scalar): values <= k stay exact,
+                    // larger ones stay above K, and the saturation period 15 - max k
+                    // keeps every stored value <= 15
+                    const int satu = (int)td.w + 1 - 2 * c;
 #pragma unroll
-                    for (int i = 0; i < BMAX; ++i) v[i] = imin(v[i], satv - i);
+                    for (int i = 0; i < BMAX; ++i) v[i] = imin(v[i], satu - i);
                 }
             } else {
                 int up = 1 << 20;

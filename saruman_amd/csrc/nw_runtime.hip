@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -642,6 +643,19 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
                 g = ge;
             }
             bc.ntiles = (uint32_t)(tiles.size() / 4) - bc.tile_begin;
+            {
+                // longest tiles first (each tile keeps one L mod 8): the persistent
+                // waves then end on the short ones, and the class's last wave
+                // finishes soon after the others
+                std::vector<std::array<uint32_t, 4>> tv(bc.ntiles);
+                std::memcpy(tv.data(), tiles.data() + 4 * (size_t)bc.tile_begin,
+                            sizeof(uint32_t) * 4 * bc.ntiles);
+                std::stable_sort(tv.begin(), tv.end(),
+                                 [](const std::array<uint32_t, 4> &x,
+                                    const std::array<uint32_t, 4> &y) { return x[2] > y[2]; });
+                std::memcpy(tiles.data() + 4 * (size_t)bc.tile_begin, tv.data(),
+                            sizeof(uint32_t) * 4 * bc.ntiles);
+            }
             // persistent grid: as many 4-wave blocks as stay resident on every CU
             // (occupancy of this kernel instance), never more waves than tiles
             int per_cu = srm::band_occupancy(bc.nw, match == 0 && mismatch == 1);
