@@ -29,15 +29,14 @@ PEAK_INT32_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz, int32 VALU ops/s
 PEAK_HBM_GBS = 8000.0
 OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 
-# HBM bytes of one step of the band kernels, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (scaled x2.000 and x1.000 by a 256 MiB copy kernel), for the exact
-# configuration profiled: workload, candidates, launch tuning and the sha256
-# prefix of the kernel source.  Source: profiles/band_kernel_mixed_2M_reads.md
+# HBM bytes of one step of the band phase (band_kernel and trace_kernel of every
+# class), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.000 and x1.000
+# by a 256 MiB copy kernel), for the exact configuration profiled: workload,
+# candidates, launch tuning and the sha256 prefix of the kernel source.
+# Source: profiles/band_and_trace_kernels_mixed_2M_reads.md
 PMC_TRAFFIC = [
-    {"workload": "mixed", "candidates": 1774238,
-     "tuning": {"blocks_per_cu": [0, 2, 0, 3, 0, 1, 1, 2, 0], "merge_below": 3,
-                "class_streams": 0},
-     "kernel_sha": "09fee5885613", "bytes": 12005253835},
+    {"workload": "mixed", "candidates": 1774238, "tuning": "defaults",
+     "kernel_sha": "e6fdbc86f66a", "bytes": 8993890778},
 ]
 
 
@@ -394,7 +393,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS,
                      "traffic": profiled_traffic(args.workload, int(db.n), tuning),
-                     "kernel": "band_kernel (all band classes of one step)",
+                     "kernel": "band_kernel + trace_kernel of every band class of one step "
+                               "(HIP events around the band phase)",
                      "ops_per_step": OPS_PER_BAND_CELL * bcells,
                      "kernel_ms": band_ms},
         "roofline_hbm": {"achieved": hbm / (band_ms / 1e3) / 1e9, "peak": PEAK_HBM_GBS,
