@@ -3,6 +3,7 @@
 // per-read parameters, the result object and the sequential batch pass.
 #pragma once
 
+#include <array>
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -28,13 +29,17 @@ inline int code_of(char c)
 
 inline char comp(char c)
 {
-    switch (c) {
-    case 'A': return 'T';
-    case 'C': return 'G';
-    case 'G': return 'C';
-    case 'T': return 'A';
-    default: return c;
-    }
+    // A<->T, C<->G, every other byte unchanged: one table load, no branch per base
+    static constexpr auto t = [] {
+        std::array<char, 256> a{};
+        for (int i = 0; i < 256; ++i) a[(size_t)i] = (char)i;
+        a['A'] = 'T';
+        a['T'] = 'A';
+        a['C'] = 'G';
+        a['G'] = 'C';
+        return a;
+    }();
+    return t[(uint8_t)c];
 }
 
 // Index of every q-gram of the genome at offsets 0..len-q (util.c:456-499), in

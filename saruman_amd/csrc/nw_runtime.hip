@@ -900,20 +900,41 @@ int srm_calibrate_copy(int device, size_t bytes, float *ms)
 
 long long srm_pack_codes(const char *seq, size_t n, uint32_t *words, size_t code_off)
 {
+    // code of every byte (A0 C1 G2 T3, anything else 0) and 1 for the bytes that
+    // are not ACGT: table loads, no branch per base
+    static constexpr auto code = [] {
+        std::array<uint8_t, 256> a{};
+        a['C'] = 1;
+        a['G'] = 2;
+        a['T'] = 3;
+        return a;
+    }();
+    static constexpr auto other = [] {
+        std::array<uint8_t, 256> a{};
+        for (auto &x : a) x = 1;
+        a['A'] = a['C'] = a['G'] = a['T'] = 0;
+        return a;
+    }();
     long long bad = 0;
-    for (size_t i = 0; i < n; ++i) {
-        uint32_t c;
-        switch (seq[i]) {
-        case 'A': c = 0; break;
-        case 'C': c = 1; break;
-        case 'G': c = 2; break;
-        case 'T': c = 3; break;
-        default: c = 0; ++bad; break;
+    auto put = [&](size_t i) {   // one code into its word, keeping the word's other codes
+        const uint8_t ch = (uint8_t)seq[i];
+        const size_t p = code_off + i;
+        const uint32_t sh = (uint32_t)(p & 15) * 2;
+        words[p >> 4] = (words[p >> 4] & ~(3u << sh)) | ((uint32_t)code[ch] << sh);
+        bad += other[ch];
+    };
+    size_t i = 0;
+    for (; i < n && ((code_off + i) & 15); ++i) put(i);
+    for (; i + 16 <= n; i += 16) {   // whole words
+        uint32_t w = 0;
+        for (int j = 0; j < 16; ++j) {
+            const uint8_t ch = (uint8_t)seq[i + (size_t)j];
+            w |= (uint32_t)code[ch] << (2 * j);
+            bad += other[ch];
         }
-        size_t p = code_off + i;
-        uint32_t sh = (uint32_t)(p & 15) * 2;
-        words[p >> 4] = (words[p >> 4] & ~(3u << sh)) | (c << sh);
+        words[(code_off + i) >> 4] = w;
     }
+    for (; i < n; ++i) put(i);
     return bad;
 }
 

@@ -86,3 +86,33 @@ def test_pack_codes_roundtrip():
     w2 = np.zeros_like(words)
     assert lib.srm_pack_codes(seq, len(seq), w2.ctypes.data, 256) == 0
     assert (w2 == words).all()
+
+
+def test_pack_codes_offsets_and_other_bytes():
+    """srm_pack_codes at every alignment: the head before a word boundary, whole
+    words and the tail; bytes other than ACGT pack as 0 and are counted; codes
+    outside [code_off, code_off + n) keep their value."""
+    lib = _lib.load()
+    rng = np.random.default_rng(9)
+    alphabet = np.frombuffer(b"ACGTNacgtRX", np.uint8)
+    table = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+    for _ in range(200):
+        n = int(rng.integers(0, 90))
+        off = int(rng.integers(0, 40))
+        p_other = float(rng.choice([0.0, 0.2]))
+        pick = np.where(rng.random(n) < p_other, rng.integers(4, len(alphabet), n),
+                        rng.integers(0, 4, n))
+        seq = alphabet[pick].tobytes()
+        nw = (off + n + 15) // 16 + 1
+        before = rng.integers(0, 2**32, nw, dtype=np.uint64).astype(np.uint32)
+        want = [int(x) for x in before]
+        bad = 0
+        for i, ch in enumerate(seq):
+            c = table.get(ch, 0)
+            bad += ch not in table
+            p = off + i
+            sh = 2 * (p & 15)
+            want[p >> 4] = (want[p >> 4] & ~(3 << sh) & 0xFFFFFFFF) | (c << sh)
+        got = before.copy()
+        assert lib.srm_pack_codes(seq, n, got.ctypes.data, off) == bad
+        assert [int(x) for x in got] == want, (n, off)
