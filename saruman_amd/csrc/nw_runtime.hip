@@ -595,11 +595,22 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     std::vector<int> cls(n, 0);
     const int merge_below = std::max(1, std::min(8, ctx->tuning.merge_below));
     for (uint32_t t : band_idx) cls[t] = std::max((band_cells(cand[t]) + 7) / 8, merge_below);
-    std::stable_sort(band_idx.begin(), band_idx.end(), [&](uint32_t a, uint32_t b) {
-        if (cls[a] != cls[b]) return cls[a] < cls[b];
-        if ((cand[a].L & 7) != (cand[b].L & 7)) return (cand[a].L & 7) < (cand[b].L & 7);
-        return cand[a].L > cand[b].L;
-    });
+    {
+        // the order of a stable sort by (class, L mod 8, L descending), from two
+        // stable counting passes: by L descending, then by (class, L mod 8)
+        uint32_t maxL = 0;
+        for (uint32_t t : band_idx) maxL = std::max<uint32_t>(maxL, cand[t].L);
+        std::vector<uint32_t> tmp(band_idx.size());
+        std::vector<size_t> at((size_t)maxL + 2, 0);
+        for (uint32_t t : band_idx) ++at[(size_t)(maxL - cand[t].L) + 1];
+        for (size_t x = 1; x < at.size(); ++x) at[x] += at[x - 1];
+        for (uint32_t t : band_idx) tmp[at[maxL - cand[t].L]++] = t;
+        std::vector<size_t> bk(9 * 8 + 1, 0);
+        auto key = [&](uint32_t t) { return (size_t)cls[t] * 8 + (cand[t].L & 7); };
+        for (uint32_t t : tmp) ++bk[key(t) + 1];
+        for (size_t x = 1; x < bk.size(); ++x) bk[x] += bk[x - 1];
+        for (uint32_t t : tmp) band_idx[bk[key(t)]++] = t;
+    }
     std::vector<uint32_t> tiles;   // uint4 per tile
     j->classes.clear();
     size_t scratch_words = 0;
@@ -674,9 +685,19 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     }
 
     // ---- plain plans
-    std::vector<uint32_t> all_idx(band_idx);
-    all_idx.insert(all_idx.end(), rest_idx.begin(), rest_idx.end());
-    std::sort(all_idx.begin(), all_idx.end());
+    // every candidate that is not on the raw-byte path, in index order
+    std::vector<uint32_t> all_idx;
+    all_idx.reserve(band_idx.size() + rest_idx.size());
+    {
+        size_t b = 0;
+        for (uint32_t t = 0; t < (uint32_t)n; ++t) {
+            if (b < bytes_idx.size() && bytes_idx[b] == t) {
+                ++b;
+                continue;
+            }
+            all_idx.push_back(t);
+        }
+    }
     std::vector<uint64_t> off_rest, off_all, off_bytes;
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);

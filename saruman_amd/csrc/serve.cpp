@@ -50,6 +50,7 @@ inline char comp(char c)
 
 struct Request {
     uint64_t id = 0;
+    size_t max_len = 0;   // longest read
     std::string reads;
     std::vector<uint64_t> off;
     std::vector<uint16_t> len;
@@ -66,7 +67,9 @@ struct Request {
     size_t ops_total = 0;
     size_t dev_bytes = 0;
     // results
-    std::string records;
+    std::string records;              // perfect hits, then (host formatting) aligned hits
+    std::unique_ptr<char[]> rec_text; // aligned hits formatted on the device (after records)
+    size_t rec_len = 0;
     srm_request_stats st{};
     bool done = false;
 };
@@ -176,6 +179,7 @@ void srm_server::prepare(Request &r)
     // per 64 candidates plus one partial tile per band class and L mod 8)
     size_t maxL = 0;
     for (auto L : r.len) maxL = std::max<size_t>(maxL, L);
+    r.max_len = maxL;
     const size_t nc = r.cand.size();
     const size_t tiles = (nc + 63) / 64 + 64;
     r.dev_bytes = r.read_words.size() * 4 + nc * (sizeof(srm_cand) + sizeof(srm_result) + 4 * 4) +
@@ -193,7 +197,8 @@ void srm_server::run_on_slot(int s, Request &r)
     const int n = (int)r.cand.size();
     int rc = SRM_OK;
     std::vector<srm_result> res((size_t)n);
-    std::vector<uint32_t> ops(r.ops_total + 1);
+    // transcripts come to the host only when the host formats the records
+    std::vector<uint32_t> ops(cfg.gpu_records ? 0 : r.ops_total + 1);
     if (n > 0) {
         rc = srm_job_upload(j, 0, 1, r.cand.data(), n, r.read_words.data(), r.read_words.size(),
                             nullptr, 0, nullptr, 0, nullptr, 0);
@@ -212,38 +217,63 @@ void srm_server::run_on_slot(int s, Request &r)
     // candidate order
     std::string out;
     std::vector<char> mapped(r.len.size(), 0);
-    char line[2 * (SRM_MAX_L + 64) + 128];
-    for (int sd = 0; sd < 2; ++sd) {
-        for (size_t p = 0; p < r.perf_read[sd].size(); ++p) {
-            const int32_t i = r.perf_read[sd][p];
-            const int64_t S = r.perf_pos[sd][p];
-            const int L = r.len[(size_t)i];
-            const char *rd = (sd ? r.reads_rev.data() : r.reads.data()) + r.off[(size_t)i];
-            int len = snprintf(line, sizeof line, "r%d\t%d\t%d\t%s\t%.*s\t%.*s\t%d\n", i, (int)S,
-                               (int)(S + L - 1), sd ? "<<" : ">>", L, genome.data() + S, L, rd, 0);
-            if (len > 0 && len < (int)sizeof line) out.append(line, (size_t)len);
-            mapped[(size_t)i] = 1;
+    {
+        // perfect hits: "r<id>\t<start>\t<stop>\t<strand>\t<genome>\t<read>\t0\n", as
+        // the reference prints them (%d fields), written directly
+        size_t bytes = 0;
+        for (int sd = 0; sd < 2; ++sd)
+            for (int32_t i : r.perf_read[sd]) bytes += 2 * (size_t)r.len[(size_t)i] + 48;
+        out.reserve(bytes);
+        auto put_int = [&](long long v) {
+            char b[24];
+            int q = 0;
+            const bool neg = v < 0;
+            unsigned long long u = neg ? 0ull - (unsigned long long)v : (unsigned long long)v;
+            do {
+                b[q++] = (char)('0' + u % 10);
+                u /= 10;
+            } while (u);
+            if (neg) out.push_back('-');
+            while (q) out.push_back(b[--q]);
+        };
+        for (int sd = 0; sd < 2; ++sd) {
+            for (size_t p = 0; p < r.perf_read[sd].size(); ++p) {
+                const int32_t i = r.perf_read[sd][p];
+                const int64_t S = r.perf_pos[sd][p];
+                const int L = r.len[(size_t)i];
+                const char *rd = (sd ? r.reads_rev.data() : r.reads.data()) + r.off[(size_t)i];
+                out.push_back('r');
+                put_int(i);
+                out.push_back('\t');
+                put_int((int)S);
+                out.push_back('\t');
+                put_int((int)(S + L - 1));
+                out.append(sd ? "\t<<\t" : "\t>>\t");
+                out.append(genome.data() + S, strnlen(genome.data() + S, (size_t)L));   // %.*s
+                out.push_back('\t');
+                out.append(rd, strnlen(rd, (size_t)L));
+                out.append("\t0\n");
+                mapped[(size_t)i] = 1;
+            }
         }
     }
     r.st.n_perfect = (long long)(r.perf_read[0].size() + r.perf_read[1].size());
     if (n > 0 && cfg.gpu_records) {
+        // the text's exact size first, then the text into a buffer of that size
+        // (no zero-filled allocation of an estimate)
         long long hits = 0;
-        const size_t base = out.size();
-        size_t cap = (size_t)n * 64;
-        for (int pass = 0; pass < 2; ++pass) {
-            out.resize(base + cap);
-            long long got = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r",
-                                            &out[base], cap, &hits);
-            if (got < 0) {
-                r.st.status = (int)got;
-                return;
-            }
-            if ((size_t)got <= cap) {
-                out.resize(base + (size_t)got);
-                break;
-            }
-            cap = (size_t)got;   // the first call was a size query
+        long long total = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r",
+                                          nullptr, 0, &hits);
+        if (total > 0) {
+            r.rec_text.reset(new char[(size_t)total]);
+            total = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r",
+                                    r.rec_text.get(), (size_t)total, &hits);
         }
+        if (total < 0) {
+            r.st.status = (int)total;
+            return;
+        }
+        r.rec_len = (size_t)total;
         r.st.n_hits = hits;
         for (int t = 0; t < n; ++t)
             if (res[(size_t)t].score <= (int)r.cand[(size_t)t].k) mapped[(size_t)r.rid[(size_t)t]] = 1;
@@ -474,8 +504,11 @@ long long srm_server_result(srm_server *S, long long id, char *out, size_t outca
     if (it == S->all.end() || !it->second->done) return SRM_EINVAL;
     auto r = it->second;
     if (st) *st = r->st;
-    long long need = (long long)r->records.size();
-    if (out && (size_t)need <= outcap) std::memcpy(out, r->records.data(), (size_t)need);
+    long long need = (long long)(r->records.size() + r->rec_len);
+    if (out && (size_t)need <= outcap) {
+        std::memcpy(out, r->records.data(), r->records.size());
+        if (r->rec_len) std::memcpy(out + r->records.size(), r->rec_text.get(), r->rec_len);
+    }
     if (release && out && (size_t)need <= outcap) S->all.erase(it);
     return need;
 }
