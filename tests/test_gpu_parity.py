@@ -111,11 +111,17 @@ def test_first_candidate_quirk(orc, kernel):
 def test_edges(orc):
     banner("[gpu] edges")
     rng = np.random.default_rng(13)
-    for n, L, k in ((1, 36, 2), (63, 1, 0), (65, 5, 0), (130, 20, 14), (64, 300, 3)):
+    # 600 bp: the longest reads the walk stages in LDS; 700 and 880 bp: the walk
+    # reads their codes from read_words
+    for n, L, k in ((1, 36, 2), (63, 1, 0), (65, 5, 0), (130, 20, 14), (64, 300, 3),
+                    (64, 600, 5), (70, 700, 5), (40, 880, 4)):
         batch = random_batch(rng, n, L=L, k=k, flags=(0, 1, 3, 5, 7))
         want = orc.align_batch(batch)
         for kernel in ("banded", "plain"):
-            check(batch, sa.align_batch(batch, kernel=kernel), want, kernel)
+            got = sa.align_batch(batch, kernel=kernel)
+            if kernel == "banded" and L >= 300:
+                assert got.paths[0] == batch.n, (L, got.paths)
+            check(batch, got, want, kernel)
 
 
 # (match, mismatch, k, window, read, flag, pos, record) worked by hand; the same
