@@ -21,7 +21,7 @@ LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libsaruman_hip.so"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["nw_kernels.hip", "nw_runtime.hip", "records.cpp", "filter.cpp", "filter_gpu.hip",
+HIP_SOURCES = ["nw_kernels.hip", "band_bits.hip", "nw_runtime.hip", "records.cpp", "filter.cpp", "filter_gpu.hip",
                "records_gpu.hip", "synth.cpp", "serve.cpp", "ingest.cpp"]
 
 HIPCC_FLAGS = [

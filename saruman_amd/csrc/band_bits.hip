@@ -1,0 +1,652 @@
+// band_bits.hip -- the banded kernels for unit costs (match 0, mismatch 1, gap 1),
+// the costs of every SARUMAN run (src/saruman.c: match 0, mismatch 1).  Same
+// results as band_kernel / trace_kernel (nw_kernels.hip), from a bit-parallel
+// forward pass.
+//
+// Forward pass (bits_band_kernel): with unit costs neighbouring DP cells differ by
+// -1, 0 or +1, so a column of the band is two bit vectors (vertical deltas +1 and
+// -1) and the next column follows from them and the column's equal-character bits
+// in a couple of dozen word operations (Myers 1999; Hyyrö 2003 for the banded,
+// diagonal-aligned form).  Bit i of a column is band cell (diagonal) i, row
+// h = c + i - k.  Cells outside the band enter as upper bounds (the cell above
+// the band as its left neighbour + 1, the cell below as the one above it + 1), so
+// every band value is >= the reference's and equal to it wherever the reference's
+// is <= k (the cells a hit's path and its comparisons use).  Rows above row 0
+// continue M[h][c] = c - h, which the recurrence reproduces and which gives row 0
+// its M[0][c] = c.
+//
+// What the walk needs is stored per block of 8 columns (columns 8q+1 .. 8q+8):
+//   - D bits: for each diagonal i and column, whether the cell equals its
+//     diagonal predecessor (D0).  8 bits per diagonal per block, the forward
+//     pass's 8 column words transposed in registers;
+//   - the block's boundary column 8q: the value of band cell 0 and the column's
+//     delta vectors.  A band value is then boundary + deltas + D bits.
+// That is 1 bit per cell plus 3-5 words per block, against 4 bits per cell for
+// the nibble band.  Columns L - 1 and L go to the tile's tail as nibbles, in the
+// format trace_kernel reads (column L has the free trailing rows).
+//
+// Walk (bits_trace_kernel): the decisions of cuda-functions.cu:121-276 with unit
+// costs.  A cell steps diagonally iff D0 is 0 (Dg + mismatch == v) or the
+// characters are equal (Dg == v), so a diagonal run is the run of cells with
+// D0 & mismatch == 0: found for a whole block from one byte.  The walk carries
+// the value v of its cell; the other cells a decision compares (Tp, Lf) are
+// computed from the block's boundary and D bits.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nw_device.h"
+#include "nw_seq.h"
+
+namespace srm {
+
+template <int NW>
+struct BitsShape {
+    static constexpr int BMAX = 8 * NW;                  // band cells stored per column
+    static constexpr int NWD = NW <= 4 ? 1 : 2;          // 32-bit words of a column vector
+    // the transposed D bits: word j of a half holds diagonals j, 8+j, 16+j, 24+j
+    // (one byte each); halves with fewer diagonals are packed into fewer words
+    static constexpr int NLO = NW <= 4 ? NW : 4;         // bytes used per word, diagonals 0-31
+    static constexpr int NHI = NW <= 4 ? 0 : NW - 4;     // diagonals 32-63
+    static constexpr int dwords(int n) { return n == 0 ? 0 : n >= 3 ? 8 : n == 2 ? 4 : 2; }
+    static constexpr int DWLO = dwords(NLO);
+    static constexpr int DW = DWLO + dwords(NHI);        // D words per block
+    static constexpr int WB = DW + 1 + 2 * NWD;          // + cell 0's value, +1 deltas, -1 deltas
+};
+
+// word and bit offset of diagonal i's D byte among a block's D words
+template <int NW>
+__device__ __forceinline__ void dloc(int i, int &word, int &shift)
+{
+    using S = BitsShape<NW>;
+    const bool hi = S::NHI > 0 && i >= 32;
+    const int n = hi ? S::NHI : S::NLO;
+    const int base = hi ? S::DWLO : 0;
+    const int ii = hi ? i - 32 : i;
+    const int j = ii & 7, g = ii >> 3;
+    if (n >= 3) {
+        word = base + j;
+        shift = 8 * g;
+    } else if (n == 2) {
+        word = base + (j & 3);
+        shift = 8 * (2 * g + (j >> 2));
+    } else {
+        word = base + (j & 1);
+        shift = 8 * (j >> 1);
+    }
+}
+
+// bits 0, 2, .., 30 of x moved to bits 0 .. 15
+__device__ __forceinline__ uint32_t even_bits(uint32_t x)
+{
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0x0000FFFFu;
+}
+
+// the 32 codes from code index idx as two bit planes (p0: low bits, p1: high bits;
+// bit b = code idx + b)
+__device__ __forceinline__ void planes32(const uint32_t *__restrict__ words, int64_t idx,
+                                         uint32_t &p0, uint32_t &p1)
+{
+    const int64_t wi = idx >> 4;
+    const uint32_t sh = (uint32_t)(idx & 15) * 2u;
+    const uint32_t a = words[wi], b = words[wi + 1], c = words[wi + 2];
+    const uint32_t x = __builtin_amdgcn_alignbit(b, a, sh);
+    const uint32_t y = __builtin_amdgcn_alignbit(c, b, sh);
+    p0 = even_bits(x) | (even_bits(y) << 16);
+    p1 = even_bits(x >> 1) | (even_bits(y >> 1) << 16);
+}
+
+// 8x8 bit transposes in each byte of x[0..7]: afterwards bit j' of byte g of x[j]
+// is what bit j of byte g of x[j'] was
+__device__ __forceinline__ void transpose8(uint32_t (&x)[8])
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t t = ((x[r] >> 4) ^ x[r + 4]) & 0x0F0F0F0Fu;
+        x[r + 4] ^= t;
+        x[r] ^= t << 4;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r += (r & 1) ? 3 : 1) {   // 0, 1, 4, 5
+        const uint32_t t = ((x[r] >> 2) ^ x[r + 2]) & 0x33333333u;
+        x[r + 2] ^= t;
+        x[r] ^= t << 2;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r += 2) {
+        const uint32_t t = ((x[r] >> 1) ^ x[r + 1]) & 0x55555555u;
+        x[r + 1] ^= t;
+        x[r] ^= t << 1;
+    }
+}
+
+template <int NW>
+struct BitsWaves {
+    static constexpr int value = NW <= 4 ? 8 : 6;
+};
+
+// Forward pass of a unit-cost band class: persistent waves pull 64-candidate
+// tiles, one candidate per lane (the same tile format and queue as band_kernel).
+template <int NW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BitsWaves<NW>::value)))
+void bits_band_kernel(BandArgs a)
+{
+    using S = BitsShape<NW>;
+    constexpr int BMAX = S::BMAX, NWD = S::NWD, DW = S::DW, WB = S::WB;
+    constexpr int WW = (BMAX + 15) / 16;   // code words of column L's band
+    const int lane = threadIdx.x & 63;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(a.queue, 1u);
+        tile = __builtin_amdgcn_readfirstlane(tile);
+        if (tile >= a.ntiles) break;
+        uint4 td = a.tiles[tile];
+        td.x = __builtin_amdgcn_readfirstlane(td.x);
+        td.y = __builtin_amdgcn_readfirstlane(td.y);
+        td.z = __builtin_amdgcn_readfirstlane(td.z);
+        uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
+        const bool active = (uint32_t)lane < td.y;
+        const uint32_t t = active ? a.perm[td.x + lane] : 0u;
+        const srm_cand cd = a.cand[t];
+        const int L = active ? (int)cd.L : 0;
+        const int w = cd.w, k = cd.k;
+        const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
+        const int dlo = -k;
+        const int K = k + 1;
+        const int Lmax = (int)td.z;
+
+        // ---- column 0: deltas of M[h][0] = max(0, h - 2s) (Q7) and of c - h above
+        // row 0: -1 for h <= 0 (cells 0..k), 0 up to row 2s, +1 below
+        uint32_t VP[NWD], VN[NWD];
+#pragma unroll
+        for (int m = 0; m < NWD; ++m) {
+            const int p0 = k + 2 * s + 1 - 32 * m;   // first +1 cell, relative to word m
+            VP[m] = p0 <= 0 ? ~0u : p0 >= 32 ? 0u : ~((1u << p0) - 1u);
+            const int n0 = k + 1 - 32 * m;           // cells below n0 are -1
+            VN[m] = n0 <= 0 ? 0u : n0 >= 32 ? ~0u : (1u << n0) - 1u;
+        }
+        int vt = k;   // value of band cell 0 (row -k: M = k)
+        auto store_boundary = [&](int q) {
+            uint32_t *__restrict__ b = scr + (size_t)q * WB * 64 + lane;
+            __builtin_nontemporal_store((uint32_t)vt, &b[DW * 64]);
+#pragma unroll
+            for (int m = 0; m < NWD; ++m) {
+                __builtin_nontemporal_store(VP[m], &b[(DW + 1 + m) * 64]);
+                __builtin_nontemporal_store(VN[m], &b[(DW + 1 + NWD + m) * 64]);
+            }
+        };
+        if (Lmax >= 2) store_boundary(0);
+
+        // window codes of band cell 0 in column c: gbase + c - 1 (cell (h, c)
+        // compares window row h - 1 with read column c - 1).  Bit planes of the
+        // codes stream through chunks of 32: ch[m] holds codes gbase + 32(m0 + m).
+        const int64_t gbase = (int64_t)cd.win_off + dlo;
+        const int64_t rbase = (int64_t)cd.read_off;
+        uint32_t ch0[NWD + 1], ch1[NWD + 1];
+#pragma unroll
+        for (int m = 0; m <= NWD; ++m) {
+            ch0[m] = 0u;
+            ch1[m] = 0u;
+            if (active) planes32(a.win_words, gbase + 32 * m, ch0[m], ch1[m]);
+        }
+        uint32_t rbuf = 0;
+        uint32_t d[8][NWD];   // D0 words of the group's 8 columns
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int m = 0; m < NWD; ++m) d[j][m] = 0u;
+
+        for (int cb0 = 1; cb0 < Lmax; cb0 += 8) {
+            // columns cb .. cb+7 = block q; cb is the same on every lane
+            const int cb = __builtin_amdgcn_readfirstlane(cb0);
+            const int q = (cb - 1) >> 3;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c = cb + j;
+                if (c < L) {
+                    const int o = c - 1;   // code offset of the column
+                    if (j == 0 && (o & 31) == 0 && o > 0) {
+#pragma unroll
+                        for (int m = 0; m < NWD; ++m) {
+                            ch0[m] = ch0[m + 1];
+                            ch1[m] = ch1[m + 1];
+                        }
+                        planes32(a.win_words, gbase + o + 32 * NWD, ch0[NWD], ch1[NWD]);
+                    }
+                    if (j == 0 && (o & 15) == 0) rbuf = fetch16(a.read_words, rbase + o);
+                    const uint32_t sh = (uint32_t)(o & 31);
+                    const uint32_t rsh = (uint32_t)(o & 15) * 2u;
+                    // the read code's bits as all-zero / all-one masks
+                    const uint32_t r0 = (uint32_t)__builtin_amdgcn_sbfe((int)rbuf, rsh, 1);
+                    const uint32_t r1 = (uint32_t)__builtin_amdgcn_sbfe((int)rbuf, rsh + 1u, 1);
+                    uint32_t ne[NWD];   // band cells whose characters differ
+#pragma unroll
+                    for (int m = 0; m < NWD; ++m) {
+                        const uint32_t g0 = __builtin_amdgcn_alignbit(ch0[m + 1], ch0[m], sh);
+                        const uint32_t g1 = __builtin_amdgcn_alignbit(ch1[m + 1], ch1[m], sh);
+                        ne[m] = (g0 ^ r0) | (g1 ^ r1);
+                    }
+                    // previous column's deltas moved to this column's rows (cell i
+                    // of column c - 1 is cell i - 1 here); the cell below the band
+                    // enters as +1
+                    uint32_t vp[NWD], vn[NWD];
+                    if (NWD == 1) {
+                        vp[0] = __builtin_amdgcn_alignbit(1u, VP[0], 1);
+                        vn[0] = VN[0] >> 1;
+                    } else {
+                        vp[0] = __builtin_amdgcn_alignbit(VP[NWD - 1], VP[0], 1);
+                        vp[NWD - 1] = __builtin_amdgcn_alignbit(1u, VP[NWD - 1], 1);
+                        vn[0] = __builtin_amdgcn_alignbit(VN[NWD - 1], VN[0], 1);
+                        vn[NWD - 1] = VN[NWD - 1] >> 1;
+                    }
+                    // D0 = (((Eq & vp) + vp) ^ vp) | Eq | vn, the add carrying across words
+                    uint32_t d0[NWD];
+                    if (NWD == 1) {
+                        const uint32_t eqp = vp[0] & ~ne[0];
+                        d0[0] = ((eqp + vp[0]) ^ vp[0]) | ~ne[0] | vn[0];
+                    } else {
+                        const uint64_t P = ((uint64_t)vp[NWD - 1] << 32) | vp[0];
+                        const uint64_t E = ~(((uint64_t)ne[NWD - 1] << 32) | ne[0]);
+                        const uint64_t N = ((uint64_t)vn[NWD - 1] << 32) | vn[0];
+                        const uint64_t D = (((E & P) + P) ^ P) | E | N;
+                        d0[0] = (uint32_t)D;
+                        d0[NWD - 1] = (uint32_t)(D >> 32);
+                    }
+                    // horizontal deltas, then this column's vertical deltas; the
+                    // cell above the band enters with a horizontal +1
+                    uint32_t hp[NWD], hn[NWD];
+#pragma unroll
+                    for (int m = 0; m < NWD; ++m) {
+                        hp[m] = vn[m] | ~(d0[m] | vp[m]);
+                        hn[m] = d0[m] & vp[m];
+                    }
+                    uint32_t xh[NWD], hns[NWD];
+                    xh[0] = (hp[0] << 1) | 1u;
+                    hns[0] = hn[0] << 1;
+                    if (NWD == 2) {
+                        xh[NWD - 1] = __builtin_amdgcn_alignbit(hp[NWD - 1], hp[0], 31);
+                        hns[NWD - 1] = __builtin_amdgcn_alignbit(hn[NWD - 1], hn[0], 31);
+                    }
+#pragma unroll
+                    for (int m = 0; m < NWD; ++m) {
+                        VN[m] = xh[m] & d0[m];
+                        VP[m] = hns[m] | ~(d0[m] | xh[m]);
+                        d[j][m] = d0[m];
+                    }
+                }
+            }
+            // ---- block q: D bits by diagonal, then the boundary of block q + 1
+            // (column cb + 7) when a later block is read
+            uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
+            uint32_t diag0 = 0;
+#pragma unroll
+            for (int m = 0; m < NWD; ++m) {
+                uint32_t x[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) x[j] = d[j][m];
+                transpose8(x);
+                if (m == 0) diag0 = x[0] & 0xFFu;   // diagonal 0, columns cb .. cb+7
+                const int n = m == 0 ? S::NLO : S::NHI;
+                const int base = m == 0 ? 0 : S::DWLO;
+                if (n >= 3) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) __builtin_nontemporal_store(x[j], &blk[(base + j) * 64]);
+                } else if (n == 2) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t y = (x[u] & 0xFFu) | ((x[u + 4] & 0xFFu) << 8) |
+                                           ((x[u] & 0xFF00u) << 8) | ((x[u + 4] & 0xFF00u) << 16);
+                        __builtin_nontemporal_store(y, &blk[(base + u) * 64]);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const uint32_t y = (x[u] & 0xFFu) | ((x[u + 2] & 0xFFu) << 8) |
+                                           ((x[u + 4] & 0xFFu) << 16) | ((x[u + 6] & 0xFFu) << 24);
+                        __builtin_nontemporal_store(y, &blk[(base + u) * 64]);
+                    }
+                }
+            }
+            // band cell 0 grows by 1 per column where its D0 bit is 0
+            const int cnt = imax(imin(L - cb, 8), 0);
+            vt += cnt - __builtin_popcount(diag0 & ((1u << cnt) - 1u));
+            if (cb + 8 < Lmax) store_boundary(q + 1);
+        }
+
+        // ---- tail for the walk's first run: columns L - 1 and L as nibbles by
+        // diagonal (nibble j of word m is cell 8m + j, values clipped at K), then
+        // the equal-character bits of column L.  Column L - 1 from its deltas;
+        // column L with the free top moves of its trailing rows.
+        uint32_t *__restrict__ tail = scr + (size_t)((Lmax + 8) >> 3) * BMAX * 64 + lane;
+        if (active) {
+            uint32_t gw[WW];
+#pragma unroll
+            for (int u = 0; u < WW; ++u) gw[u] = fetch16(a.win_words, gbase + (L - 1) + 16 * u);
+            const uint32_t rc = code_at(a.read_words, rbase + L - 1);
+            const uint32_t rrep = rc * 0x55555555u;
+            const int ifree = w - 2 * e + 1 - L - dlo;   // first cell with a free top move
+            const int iend = w - L - dlo;                // cell of row w
+            int score = K;
+            int cur = vt;                                // column L - 1, cell i
+            int up = 1 << 20;                            // column L, cell i - 1
+            uint32_t eqw[2] = {0u, 0u};
+#pragma unroll
+            for (int m = 0; m < NW; ++m) {
+                uint32_t xp = 0, xl = 0;
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) {
+                    const int i = 8 * m + jj;
+                    int nxt = 1 << 20;                   // column L - 1, cell i + 1
+                    if (i + 1 < 32 * NWD) {
+                        const int b = i + 1;
+                        nxt = cur + (int)((VP[b >> 5] >> (b & 31)) & 1u) -
+                              (int)((VN[b >> 5] >> (b & 31)) & 1u);
+                    }
+                    const uint32_t x = (gw[i >> 4] ^ rrep) >> (2 * (i & 15));
+                    const int mis = (int)((x | (x >> 1)) & 1u);
+                    int nv = imin(cur + mis, nxt + 1);
+                    nv = imin(nv, up + (i >= ifree ? 0 : 1));
+                    xp |= (uint32_t)imin(cur, K) << (4 * jj);
+                    xl |= (uint32_t)imin(nv, K) << (4 * jj);
+                    if (!mis) eqw[i >> 5] |= 1u << (i & 31);
+                    if (i == iend) score = imin(nv, K);
+                    up = nv;
+                    cur = nxt;
+                }
+                __builtin_nontemporal_store(xl, &tail[m * 64]);
+                __builtin_nontemporal_store(xp, &tail[(NW + m) * 64]);
+            }
+            tail[(2 * NW) * 64] = eqw[0];
+            tail[(2 * NW + 1) * 64] = eqw[1];
+            // ---- score (cuda-functions.cu:495-497); the walk runs in bits_trace_kernel
+            srm_result r;
+            r.score = (int16_t)score;
+            r.n_ops = 0;
+            r.end_row = (uint16_t)w;
+            r.end_col = (uint16_t)L;
+            a.res[t] = r;
+        }
+    }
+}
+
+// Traceback of every hit of a unit-cost band class (cuda-functions.cu:504-534),
+// one wave per tile of bits_band_kernel.  The first run (column L's free rows)
+// and column 0 are as in trace_kernel; elsewhere the walk goes block by block.
+template <int NW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+void bits_trace_kernel(BandArgs a)
+{
+    using S = BitsShape<NW>;
+    constexpr int BMAX = S::BMAX, NWD = S::NWD, DW = S::DW, WB = S::WB;
+    const int lane = threadIdx.x & 63;
+    const uint32_t tile = (uint32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (tile >= a.ntiles) return;
+    const uint4 td = a.tiles[tile];
+    if ((uint32_t)lane >= td.y) return;
+    const uint32_t t = a.perm[td.x + lane];
+    const srm_cand cd = a.cand[t];
+    srm_result r = a.res[t];
+    const int k = cd.k, K = k + 1;
+    if (r.score > k) return;
+    const int L = cd.L, w = cd.w;
+    const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
+    const int dlo = -k;
+    const uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
+    const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
+    uint32_t *__restrict__ ops = a.ops + cd.ops_off;
+    const int64_t rbase = (int64_t)cd.read_off;
+    const int cap = w + k;
+    int row = w, col = L, n = 0;
+    uint32_t opw = 0;
+    bool done = false;
+    int v = 0;   // value of the walk's cell once it has left column L (<= k: exact)
+    auto emit = [&](int op) {
+        opw |= (uint32_t)op << ((n & 15) * 2);
+        ++n;
+        if ((n & 15) == 0) {
+            ops[(n >> 4) - 1] = opw;
+            opw = 0;
+        }
+        row -= (op != 2);
+        col -= (op != 1);
+    };
+    {
+        // the first run, up column L (its free tail rows), from the packed tail
+        const uint32_t *__restrict__ tail =
+            scr + (size_t)((td.z + 8) >> 3) * BMAX * 64 + lane;
+        typedef uint32_t TailWords __attribute__((ext_vector_type(NW)));
+        TailWords tl, tp;
+#pragma unroll
+        for (int m = 0; m < NW; ++m) {
+            tl[m] = tail[m * 64];
+            tp[m] = tail[(NW + m) * 64];
+        }
+        const uint32_t eq0 = tail[(2 * NW) * 64], eq1 = tail[(2 * NW + 1) * 64];
+        auto nib = [&](const TailWords &X, int i) -> int {
+            if (i < 0 || i >= BMAX) return 15;
+            return (int)((X[i >> 3] >> ((i & 7) * 4)) & 15u);
+        };
+        const int wd = L - 1;
+        while (col == L) {
+            if (stop_row0 && row == 0) {
+                done = true;
+                break;
+            }
+            const int i = row - L - dlo;
+            const int vv = nib(tl, i);
+            const int Dg = row >= 1 ? imin(nib(tp, i), K) : SRM_FOREIGN;
+            const int Lf = imin(nib(tp, i + 1), K);
+            const int Tp = row >= 1 ? imin(nib(tl, i - 1), K) : SRM_FOREIGN;
+            const bool eq = i >= 0 && i < BMAX && ((i < 32 ? eq0 >> i : eq1 >> (i - 32)) & 1u);
+            const int op = decide(0, 1, vv, Dg, Lf, Tp, wd, imax(row - 1, 0), L, w, s, e, eq);
+            if (op == 3 || n >= cap) {
+                done = true;
+                break;
+            }
+            emit(op);
+        }
+        if (!done) v = nib(tp, row - (L - 1) - dlo);   // the cell in column L - 1
+    }
+    // the read's codes, staged in this wave's LDS (lane-interleaved words); word u
+    // holds the read_words word ((rbase - 1) >> 4) + u
+    extern __shared__ uint32_t trace_lds[];
+    const int rwn = a.code_words;
+    uint32_t *__restrict__ lcodes = trace_lds + (size_t)(threadIdx.x >> 6) * 64 * rwn + lane;
+    if (rwn && !done) {
+        const uint32_t *__restrict__ src = a.read_words + ((rbase - 1) >> 4);
+        const int nwd = ((L + 15) >> 4) + 3;   // <= rwn: L <= the class's max L
+        for (int u = 0; u < nwd; ++u) lcodes[u * 64] = src[u];
+    }
+    const int64_t woff = (int64_t)cd.win_off;
+    int ci = -1, cq = -1;
+    bool side = false;
+    uint32_t db = 0, mis = 0;   // D0 bits and character mismatches of the block's cells
+    uint32_t dm = 0, dp = 0;    // D0 bits of diagonals ci - 1 and ci + 1
+    int bvt = 0;                // the block's boundary: value of band cell 0, deltas
+    uint32_t bvp[NWD], bvn[NWD];
+#pragma unroll
+    for (int m = 0; m < NWD; ++m) bvp[m] = bvn[m] = 0u;
+    auto dbyte = [&](const uint32_t *__restrict__ blk, int i) -> uint32_t {
+        if (i < 0 || i >= BMAX) return 0u;
+        int wd, sh;
+        dloc<NW>(i, wd, sh);
+        return (blk[wd * 64] >> sh) & 0xFFu;
+    };
+    // value of band cell i in column 8q + nb (nb = 0 .. 8) of the current block;
+    // K outside the band (never equal to a value <= k)
+    auto cell = [&](int i, uint32_t dbits, int nb) -> int {
+        if (i < 0 || i >= BMAX) return K;
+        int base = bvt;
+#pragma unroll
+        for (int m = 0; m < NWD; ++m) {
+            // cells 1 .. i of word m
+            const int top = i - 32 * m;
+            uint32_t msk = top < 0 ? 0u : top >= 31 ? ~0u : (2u << top) - 1u;
+            if (m == 0) msk &= ~1u;
+            base += __builtin_popcount(bvp[m] & msk) - __builtin_popcount(bvn[m] & msk);
+        }
+        return base + nb - __builtin_popcount(dbits & ((1u << nb) - 1u));
+    };
+    while (!done) {
+        if (stop_row0 && row == 0) break;
+        if (col == 0) {
+            // column 0: M[h][0] = max(0, h - 2s) (Q7); a vertical step or the stop
+            // remain (Dg = Lf = COL0)
+            const int vv = imax(row - 2 * s, 0);
+            const int Tp = row >= 1 ? imax(row - 1 - 2 * s, 0) : SRM_FOREIGN;
+            const int op = decide(0, 1, vv, SRM_COL0, SRM_COL0, Tp, 0, imax(row - 1, 0), L, w,
+                                  s, e, false);
+            if (op == 3 || n >= cap) break;
+            emit(op);
+            continue;
+        }
+        const int i = row - col - dlo;
+        const int q = (col - 1) >> 3, j0 = (col - 1) & 7;
+        const uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
+        if (i != ci || q != cq) {
+            db = dbyte(blk, i);
+            // characters of the block's cells: read codes 8q .. 8q+7 against window
+            // codes from i + dlo + 8q (exact for rows and columns >= 1, the only
+            // cells whose characters can decide)
+            uint32_t rc;
+            if (rwn) {   // code rbase + 8q sits at place o of the staged codes
+                const int o = (int)((rbase - 1) & 15) + 1 + 8 * q;
+                const int wi = o >> 4;
+                rc = __builtin_amdgcn_alignbit(lcodes[(wi + 1) * 64], lcodes[wi * 64],
+                                               (uint32_t)(o & 15) * 2u);
+            } else {
+                rc = fetch16(a.read_words, rbase + 8 * q);
+            }
+            const uint32_t gc = fetch16(a.win_words, woff + (i + dlo + 8 * q));
+            const uint32_t x = rc ^ gc;
+            uint32_t m8 = (x | (x >> 1)) & 0x5555u;
+            m8 = (m8 | (m8 >> 1)) & 0x3333u;
+            m8 = (m8 | (m8 >> 2)) & 0x0F0Fu;
+            mis = (m8 | (m8 >> 4)) & 0xFFu;
+            ci = i;
+            cq = q;
+            side = false;
+        }
+        // cells j0, j0-1, .. of the block step diagonally until one has D0 set
+        // and differing characters
+        const uint32_t below = (2u << j0) - 1u;
+        const uint32_t fail = db & mis & below;
+        int rr = fail ? j0 - (31 - __builtin_clz(fail)) : j0 + 1;
+        rr = imin(rr, imin(row, cap - n));
+        if (rr > 0) {
+            // rr diagonal steps: 0 bits in the transcript; v drops by the mismatches
+            const uint32_t run = below & ~((1u << (j0 + 1 - rr)) - 1u);
+            v -= __builtin_popcount(~db & run);
+            if (((n & 15) + rr) >= 16) {
+                ops[n >> 4] = opw;
+                opw = 0;
+            }
+            n += rr;
+            row -= rr;
+            col -= rr;
+            continue;
+        }
+        // one cell that is not on a diagonal run (col >= 1)
+        if (!side) {
+            dm = dbyte(blk, i - 1);
+            dp = dbyte(blk, i + 1);
+            bvt = (int)blk[DW * 64];
+#pragma unroll
+            for (int m = 0; m < NWD; ++m) {
+                bvp[m] = blk[(DW + 1 + m) * 64];
+                bvn[m] = blk[(DW + 1 + NWD + m) * 64];
+            }
+            side = true;
+        }
+        const int Dg = row >= 1 ? v - 1 + (int)((db >> j0) & 1u) : SRM_FOREIGN;
+        const int Tp = row >= 1 ? cell(i - 1, dm, j0 + 1) : SRM_FOREIGN;
+        const int Lf = cell(i + 1, dp, j0);
+        const bool eq = ((mis >> j0) & 1u) == 0u;   // decides only with row >= 1
+        const int op = decide(0, 1, v, Dg, Lf, Tp, col - 1, imax(row - 1, 0), L, w, s, e, eq);
+        if (op == 3 || n >= cap) break;
+        v = op == 0 ? Dg : op == 1 ? Tp : Lf;
+        emit(op);
+    }
+    if (n & 15) ops[n >> 4] = opw;
+    r.n_ops = (uint16_t)n;
+    r.end_row = (uint16_t)row;
+    r.end_col = (uint16_t)col;
+    a.res[t] = r;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers
+// ---------------------------------------------------------------------------
+template <int NW>
+static hipError_t launch_bits_band_nw(const BandArgs &a, int blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(bits_band_kernel<NW>, dim3(blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, hipStream_t st)
+{
+    switch (nw) {
+    case 1: return launch_bits_band_nw<1>(a, blocks, st);
+    case 2: return launch_bits_band_nw<2>(a, blocks, st);
+    case 3: return launch_bits_band_nw<3>(a, blocks, st);
+    case 4: return launch_bits_band_nw<4>(a, blocks, st);
+    case 5: return launch_bits_band_nw<5>(a, blocks, st);
+    case 6: return launch_bits_band_nw<6>(a, blocks, st);
+    case 7: return launch_bits_band_nw<7>(a, blocks, st);
+    case 8: return launch_bits_band_nw<8>(a, blocks, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NW>
+static int bits_occupancy_nw()
+{
+    int blocks = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks, (const void *)bits_band_kernel<NW>, 256, 0);
+    return e == hipSuccess && blocks > 0 ? blocks : 1;
+}
+
+int bits_band_occupancy(int nw)
+{
+    switch (nw) {
+    case 1: return bits_occupancy_nw<1>();
+    case 2: return bits_occupancy_nw<2>();
+    case 3: return bits_occupancy_nw<3>();
+    case 4: return bits_occupancy_nw<4>();
+    case 5: return bits_occupancy_nw<5>();
+    case 6: return bits_occupancy_nw<6>();
+    case 7: return bits_occupancy_nw<7>();
+    case 8: return bits_occupancy_nw<8>();
+    default: return 1;
+    }
+}
+
+template <int NW>
+static hipError_t launch_bits_trace_nw(const BandArgs &a, size_t lds, hipStream_t st)
+{
+    hipLaunchKernelGGL(bits_trace_kernel<NW>, dim3((a.ntiles + 3) / 4), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bits_trace(const BandArgs &a, int nw, size_t lds, hipStream_t st)
+{
+    switch (nw) {
+    case 1: return launch_bits_trace_nw<1>(a, lds, st);
+    case 2: return launch_bits_trace_nw<2>(a, lds, st);
+    case 3: return launch_bits_trace_nw<3>(a, lds, st);
+    case 4: return launch_bits_trace_nw<4>(a, lds, st);
+    case 5: return launch_bits_trace_nw<5>(a, lds, st);
+    case 6: return launch_bits_trace_nw<6>(a, lds, st);
+    case 7: return launch_bits_trace_nw<7>(a, lds, st);
+    case 8: return launch_bits_trace_nw<8>(a, lds, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace srm

@@ -18,48 +18,9 @@
 #include <stdint.h>
 
 #include "nw_device.h"
+#include "nw_seq.h"
 
 namespace srm {
-
-// ---------------------------------------------------------------------------
-// sequence access
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t code_at(const uint32_t *__restrict__ words, int64_t idx)
-{
-    return (words[idx >> 4] >> ((uint32_t)(idx & 15) * 2u)) & 3u;
-}
-
-// 16 consecutive codes starting at code index idx (idx may be negative down to
-// -SRM_CODE_PAD: the arrays carry padding)
-__device__ __forceinline__ uint32_t fetch16(const uint32_t *__restrict__ words, int64_t idx)
-{
-    int64_t wi = idx >> 4;
-    uint32_t sh = (uint32_t)(idx & 15) * 2u;
-    uint32_t lo = words[wi];
-    uint32_t hi = words[wi + 1];
-    return __builtin_amdgcn_alignbit(hi, lo, sh);
-}
-
-// cuda-functions.cu:383-410: flags 2,3,6,7 free the start (s = k), flags 1,3,5,7
-// free the end (e = k); every other value leaves both at 0
-__device__ __forceinline__ int flag_s(int flag, int k) { return (flag <= 7 && (flag & 2)) ? k : 0; }
-__device__ __forceinline__ int flag_e(int flag, int k) { return (flag <= 7 && (flag & 1)) ? k : 0; }
-
-__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
-__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-
-// value-based traceback decision, cuda-functions.cu:121-276 (matrix-relative).
-// returns 0 diag, 1 top, 2 left, 3 stop
-__device__ __forceinline__ int decide(int ma, int mm, int v, int Dg, int Lf, int Tp, int wd, int ht,
-                                      int L, int w, int s, int e, bool chars_equal)
-{
-    int corr = ((wd == L - 1 && ht >= w - 2 * e) || (s && wd == 0 && ht <= (s + 1) * 2)) ? mm : 0;
-    if (Dg + mm == v) return 0;
-    if (Dg + ma == v && chars_equal) return 0;
-    if (Tp + mm - corr == v) return 1;
-    if (Lf + mm == v) return 2;
-    return 3;
-}
 
 // ---------------------------------------------------------------------------
 // plain kernel
@@ -704,66 +665,59 @@ hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st)
 }
 
 template <int NW>
-static hipError_t launch_band_nw(const BandArgs &a, bool unit, int blocks, hipStream_t st)
+static hipError_t launch_band_nw(const BandArgs &a, int blocks, hipStream_t st)
 {
-    dim3 block(256), grid(blocks);
-    if (unit)
-        hipLaunchKernelGGL((band_kernel<NW, true>), grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL((band_kernel<NW, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((band_kernel<NW, false>), dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 template <int NW>
-static int occupancy_nw(bool unit)
+static int occupancy_nw()
 {
     int blocks = 0;
-    hipError_t e = unit ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &blocks, (const void *)band_kernel<NW, true>, 256, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &blocks, (const void *)band_kernel<NW, false>, 256, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks, (const void *)band_kernel<NW, false>, 256, 0);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
-// resident 256-thread blocks per CU of band_kernel<nw, unit>
+// resident 256-thread blocks per CU of the forward pass of a band class (unit
+// costs: bits_band_kernel)
 int band_occupancy(int nw, bool unit)
 {
+    if (unit) return bits_band_occupancy(nw);
     switch (nw) {
-    case 1: return occupancy_nw<1>(unit);
-    case 2: return occupancy_nw<2>(unit);
-    case 3: return occupancy_nw<3>(unit);
-    case 4: return occupancy_nw<4>(unit);
-    case 5: return occupancy_nw<5>(unit);
-    case 6: return occupancy_nw<6>(unit);
-    case 7: return occupancy_nw<7>(unit);
-    case 8: return occupancy_nw<8>(unit);
+    case 1: return occupancy_nw<1>();
+    case 2: return occupancy_nw<2>();
+    case 3: return occupancy_nw<3>();
+    case 4: return occupancy_nw<4>();
+    case 5: return occupancy_nw<5>();
+    case 6: return occupancy_nw<6>();
+    case 7: return occupancy_nw<7>();
+    case 8: return occupancy_nw<8>();
     default: return 1;
     }
 }
 
 hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st)
 {
+    if (unit) return launch_bits_band(a, nw, blocks, st);
     switch (nw) {
-    case 1: return launch_band_nw<1>(a, unit, blocks, st);
-    case 2: return launch_band_nw<2>(a, unit, blocks, st);
-    case 3: return launch_band_nw<3>(a, unit, blocks, st);
-    case 4: return launch_band_nw<4>(a, unit, blocks, st);
-    case 5: return launch_band_nw<5>(a, unit, blocks, st);
-    case 6: return launch_band_nw<6>(a, unit, blocks, st);
-    case 7: return launch_band_nw<7>(a, unit, blocks, st);
-    case 8: return launch_band_nw<8>(a, unit, blocks, st);
+    case 1: return launch_band_nw<1>(a, blocks, st);
+    case 2: return launch_band_nw<2>(a, blocks, st);
+    case 3: return launch_band_nw<3>(a, blocks, st);
+    case 4: return launch_band_nw<4>(a, blocks, st);
+    case 5: return launch_band_nw<5>(a, blocks, st);
+    case 6: return launch_band_nw<6>(a, blocks, st);
+    case 7: return launch_band_nw<7>(a, blocks, st);
+    case 8: return launch_band_nw<8>(a, blocks, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <int NW>
-static hipError_t launch_trace_nw(const BandArgs &a, bool unit, size_t lds, hipStream_t st)
+static hipError_t launch_trace_nw(const BandArgs &a, size_t lds, hipStream_t st)
 {
-    dim3 block(256), grid((a.ntiles + 3) / 4);
-    if (unit)
-        hipLaunchKernelGGL((trace_kernel<NW, true>), grid, block, lds, st, a);
-    else
-        hipLaunchKernelGGL((trace_kernel<NW, false>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((trace_kernel<NW, false>), dim3((a.ntiles + 3) / 4), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
@@ -780,15 +734,16 @@ hipError_t launch_trace(const BandArgs &args, int nw, bool unit, int maxL, hipSt
     BandArgs a = args;
     a.code_words = maxL <= 600 ? trace_code_words(maxL) : 0;
     const size_t lds = (size_t)a.code_words * 256 * 4;
+    if (unit) return launch_bits_trace(a, nw, lds, st);
     switch (nw) {
-    case 1: return launch_trace_nw<1>(a, unit, lds, st);
-    case 2: return launch_trace_nw<2>(a, unit, lds, st);
-    case 3: return launch_trace_nw<3>(a, unit, lds, st);
-    case 4: return launch_trace_nw<4>(a, unit, lds, st);
-    case 5: return launch_trace_nw<5>(a, unit, lds, st);
-    case 6: return launch_trace_nw<6>(a, unit, lds, st);
-    case 7: return launch_trace_nw<7>(a, unit, lds, st);
-    case 8: return launch_trace_nw<8>(a, unit, lds, st);
+    case 1: return launch_trace_nw<1>(a, lds, st);
+    case 2: return launch_trace_nw<2>(a, lds, st);
+    case 3: return launch_trace_nw<3>(a, lds, st);
+    case 4: return launch_trace_nw<4>(a, lds, st);
+    case 5: return launch_trace_nw<5>(a, lds, st);
+    case 6: return launch_trace_nw<6>(a, lds, st);
+    case 7: return launch_trace_nw<7>(a, lds, st);
+    case 8: return launch_trace_nw<8>(a, lds, st);
     default: return hipErrorInvalidValue;
     }
 }
