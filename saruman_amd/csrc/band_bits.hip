@@ -85,18 +85,22 @@ __device__ __forceinline__ uint32_t even_bits(uint32_t x)
     return (x | (x >> 8)) & 0x0000FFFFu;
 }
 
-// the 32 codes from code index idx as two bit planes (p0: low bits, p1: high bits;
-// bit b = code idx + b)
+// 32 codes (x: the first 16, y: the next 16) as two bit planes (p0: low bits,
+// p1: high bits; bit b = code b)
+__device__ __forceinline__ void planes_of(uint32_t x, uint32_t y, uint32_t &p0, uint32_t &p1)
+{
+    p0 = even_bits(x) | (even_bits(y) << 16);
+    p1 = even_bits(x >> 1) | (even_bits(y >> 1) << 16);
+}
+
+// the 32 codes from code index idx as two bit planes
 __device__ __forceinline__ void planes32(const uint32_t *__restrict__ words, int64_t idx,
                                          uint32_t &p0, uint32_t &p1)
 {
     const int64_t wi = idx >> 4;
     const uint32_t sh = (uint32_t)(idx & 15) * 2u;
     const uint32_t a = words[wi], b = words[wi + 1], c = words[wi + 2];
-    const uint32_t x = __builtin_amdgcn_alignbit(b, a, sh);
-    const uint32_t y = __builtin_amdgcn_alignbit(c, b, sh);
-    p0 = even_bits(x) | (even_bits(y) << 16);
-    p1 = even_bits(x >> 1) | (even_bits(y >> 1) << 16);
+    planes_of(__builtin_amdgcn_alignbit(b, a, sh), __builtin_amdgcn_alignbit(c, b, sh), p0, p1);
 }
 
 // 8x8 bit transposes in each byte of x[0..7]: afterwards bit j' of byte g of x[j]
@@ -128,8 +132,10 @@ struct BitsWaves {
     static constexpr int value = NW <= 4 ? 8 : 6;
 };
 
-// Forward pass of a unit-cost band class: persistent waves pull 64-candidate
-// tiles, one candidate per lane (the same tile format and queue as band_kernel).
+// Forward pass of a unit-cost band class: one wave per 64-candidate tile, one
+// candidate per lane (the tile format of band_kernel).  Not persistent: the
+// column loop is short, and tiles taken from one global counter serialise on its
+// atomics; the workgroup dispatcher hands out the tiles (longest first) instead.
 template <int NW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BitsWaves<NW>::value)))
 void bits_band_kernel(BandArgs a)
@@ -139,11 +145,10 @@ void bits_band_kernel(BandArgs a)
     constexpr int WW = (BMAX + 15) / 16;   // code words of column L's band
     const int lane = threadIdx.x & 63;
 
-    for (;;) {
-        uint32_t tile = 0;
-        if (lane == 0) tile = atomicAdd(a.queue, 1u);
-        tile = __builtin_amdgcn_readfirstlane(tile);
-        if (tile >= a.ntiles) break;
+    {
+        const uint32_t tile =
+            __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+        if (tile >= a.ntiles) return;
         uint4 td = a.tiles[tile];
         td.x = __builtin_amdgcn_readfirstlane(td.x);
         td.y = __builtin_amdgcn_readfirstlane(td.y);
@@ -179,7 +184,6 @@ void bits_band_kernel(BandArgs a)
                 __builtin_nontemporal_store(VN[m], &b[(DW + 1 + NWD + m) * 64]);
             }
         };
-        if (Lmax >= 2) store_boundary(0);
 
         // window codes of band cell 0 in column c: gbase + c - 1 (cell (h, c)
         // compares window row h - 1 with read column c - 1).  Bit planes of the
@@ -194,6 +198,16 @@ void bits_band_kernel(BandArgs a)
             if (active) planes32(a.win_words, gbase + 32 * m, ch0[m], ch1[m]);
         }
         uint32_t rbuf = 0;
+        // the next refills, loaded one block ahead (before the block's stores, so
+        // waiting for them never waits for the stores): raw read code words for
+        // the next 16 columns, raw window code words for the next chunk
+        const uint32_t rsh0 = (uint32_t)(rbase & 15) * 2u;   // code offset of rbase + 16m
+        const uint32_t gsh0 = (uint32_t)(gbase & 15) * 2u;   // and of gbase + 32m
+        uint32_t rr0 = 0, rr1 = 0, pg0 = 0, pg1 = 0, pg2 = 0;
+        if (active) {
+            rr0 = a.read_words[rbase >> 4];
+            rr1 = a.read_words[(rbase >> 4) + 1];
+        }
         uint32_t d[8][NWD];   // D0 words of the group's 8 columns
 #pragma unroll
         for (int j = 0; j < 8; ++j)
@@ -204,20 +218,27 @@ void bits_band_kernel(BandArgs a)
             // columns cb .. cb+7 = block q; cb is the same on every lane
             const int cb = __builtin_amdgcn_readfirstlane(cb0);
             const int q = (cb - 1) >> 3;
+            // refills from the loads issued one block earlier, on every lane (lanes
+            // past their read decode words they never use)
+            if (((cb - 1) & 31) == 0 && cb > 1) {
+#pragma unroll
+                for (int m = 0; m < NWD; ++m) {
+                    ch0[m] = ch0[m + 1];
+                    ch1[m] = ch1[m + 1];
+                }
+                planes_of(__builtin_amdgcn_alignbit(pg1, pg0, gsh0),
+                          __builtin_amdgcn_alignbit(pg2, pg1, gsh0), ch0[NWD], ch1[NWD]);
+            }
+            if (((cb - 1) & 15) == 0) rbuf = __builtin_amdgcn_alignbit(rr1, rr0, rsh0);
+            store_boundary(q);   // column cb - 1 = 8q (garbage on lanes past their read)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
+                // every lane steps every column of the tile (no branch per column);
+                // a lane past its read keeps the deltas of its column L - 1
                 const int c = cb + j;
-                if (c < L) {
+                const bool live = c < L;
+                {
                     const int o = c - 1;   // code offset of the column
-                    if (j == 0 && (o & 31) == 0 && o > 0) {
-#pragma unroll
-                        for (int m = 0; m < NWD; ++m) {
-                            ch0[m] = ch0[m + 1];
-                            ch1[m] = ch1[m + 1];
-                        }
-                        planes32(a.win_words, gbase + o + 32 * NWD, ch0[NWD], ch1[NWD]);
-                    }
-                    if (j == 0 && (o & 15) == 0) rbuf = fetch16(a.read_words, rbase + o);
                     const uint32_t sh = (uint32_t)(o & 31);
                     const uint32_t rsh = (uint32_t)(o & 15) * 2u;
                     // the read code's bits as all-zero / all-one masks
@@ -273,14 +294,31 @@ void bits_band_kernel(BandArgs a)
                     }
 #pragma unroll
                     for (int m = 0; m < NWD; ++m) {
-                        VN[m] = xh[m] & d0[m];
-                        VP[m] = hns[m] | ~(d0[m] | xh[m]);
+                        VN[m] = live ? xh[m] & d0[m] : VN[m];
+                        VP[m] = live ? hns[m] | ~(d0[m] | xh[m]) : VP[m];
                         d[j][m] = d0[m];
                     }
                 }
             }
-            // ---- block q: D bits by diagonal, then the boundary of block q + 1
-            // (column cb + 7) when a later block is read
+            // loads for the next block's refills (columns from cb + 8), issued before
+            // this block's stores
+            {
+                const int o2 = cb + 7;
+                if (cb + 8 < L) {
+                    if ((o2 & 15) == 0) {
+                        const int64_t wi = (rbase + o2) >> 4;
+                        rr0 = a.read_words[wi];
+                        rr1 = a.read_words[wi + 1];
+                    }
+                    if ((o2 & 31) == 0) {
+                        const int64_t wi = (gbase + o2 + 32 * NWD) >> 4;
+                        pg0 = a.win_words[wi];
+                        pg1 = a.win_words[wi + 1];
+                        pg2 = a.win_words[wi + 2];
+                    }
+                }
+            }
+            // ---- block q: D bits by diagonal
             uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
             uint32_t diag0 = 0;
 #pragma unroll
@@ -314,7 +352,6 @@ void bits_band_kernel(BandArgs a)
             // band cell 0 grows by 1 per column where its D0 bit is 0
             const int cnt = imax(imin(L - cb, 8), 0);
             vt += cnt - __builtin_popcount(diag0 & ((1u << cnt) - 1u));
-            if (cb + 8 < Lmax) store_boundary(q + 1);
         }
 
         // ---- tail for the walk's first run: columns L - 1 and L as nibbles by

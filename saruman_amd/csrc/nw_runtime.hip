@@ -667,13 +667,18 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
                 std::memcpy(tiles.data() + 4 * (size_t)bc.tile_begin, tv.data(),
                             sizeof(uint32_t) * 4 * bc.ntiles);
             }
-            // persistent grid: as many 4-wave blocks as stay resident on every CU
-            // (occupancy of this kernel instance), never more waves than tiles
-            int per_cu = srm::band_occupancy(bc.nw, match == 0 && mismatch == 1);
-            const int tuned = ctx->tuning.blocks_per_cu[bc.nw];
-            if (tuned > 0) per_cu = std::min(per_cu, tuned);
-            int max_blocks = std::max(1, ctx->num_cus * per_cu);
-            bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
+            if (match == 0 && mismatch == 1) {
+                // bits_band_kernel: one wave per tile
+                bc.blocks = (int)((bc.ntiles + 3) / 4);
+            } else {
+                // persistent grid: as many 4-wave blocks as stay resident on every
+                // CU (occupancy of this kernel instance), never more waves than tiles
+                int per_cu = srm::band_occupancy(bc.nw, false);
+                const int tuned = ctx->tuning.blocks_per_cu[bc.nw];
+                if (tuned > 0) per_cu = std::min(per_cu, tuned);
+                int max_blocks = std::max(1, ctx->num_cus * per_cu);
+                bc.blocks = (int)std::min<size_t>((bc.ntiles + 3) / 4, (size_t)max_blocks);
+            }
             // every tile keeps its band until trace_kernel has walked it
             for (uint32_t u = 0; u < bc.ntiles; ++u) {
                 tile_off.push_back(scratch_words);
