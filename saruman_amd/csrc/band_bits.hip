@@ -414,7 +414,7 @@ void bits_band_kernel(BandArgs a)
 // one wave per tile of bits_band_kernel.  The first run (column L's free rows)
 // and column 0 are as in trace_kernel; elsewhere the walk goes block by block.
 template <int NW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NW <= 4 ? 8 : 7)))
 void bits_trace_kernel(BandArgs a)
 {
     using S = BitsShape<NW>;
@@ -499,11 +499,18 @@ void bits_trace_kernel(BandArgs a)
         for (int u = 0; u < nwd; ++u) lcodes[u * 64] = src[u];
     }
     const int64_t woff = (int64_t)cd.win_off;
-    int ci = -1, cq = -1;
-    bool side = false;
-    uint32_t db = 0, mis = 0;   // D0 bits and character mismatches of the block's cells
-    uint32_t dm = 0, dp = 0;    // D0 bits of diagonals ci - 1 and ci + 1
-    int bvt = 0;                // the block's boundary: value of band cell 0, deltas
+    // the walk goes by segments of 32 columns (4 blocks) along its diagonal: the D0
+    // bits of the diagonal in columns 32g+1 .. 32g+32 and the mismatches of those
+    // cells, bit j for column 32g + 1 + j; a diagonal run crosses a segment in one
+    // step and waits for memory once per segment
+    int ci = -1, cg = -1;
+    uint32_t dseg = 0, mseg = 0;
+    const int qlast = (L - 2) >> 3;   // last block holding columns of this read
+    // the side cells of a non-diagonal step (block sq of diagonal si): D0 bits of
+    // diagonals si - 1 and si + 1 and the block's boundary
+    int si = -1, sq = -1;
+    uint32_t dm = 0, dp = 0;
+    int bvt = 0;                // value of band cell 0 in the boundary column
     uint32_t bvp[NWD], bvn[NWD];
 #pragma unroll
     for (int m = 0; m < NWD; ++m) bvp[m] = bvn[m] = 0u;
@@ -513,7 +520,7 @@ void bits_trace_kernel(BandArgs a)
         dloc<NW>(i, wd, sh);
         return (blk[wd * 64] >> sh) & 0xFFu;
     };
-    // value of band cell i in column 8q + nb (nb = 0 .. 8) of the current block;
+    // value of band cell i in column 8sq + nb (nb = 0 .. 8) of the side block;
     // K outside the band (never equal to a value <= k)
     auto cell = [&](int i, uint32_t dbits, int nb) -> int {
         if (i < 0 || i >= BMAX) return K;
@@ -542,53 +549,76 @@ void bits_trace_kernel(BandArgs a)
             continue;
         }
         const int i = row - col - dlo;
-        const int q = (col - 1) >> 3, j0 = (col - 1) & 7;
-        const uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
-        if (i != ci || q != cq) {
-            db = dbyte(blk, i);
-            // characters of the block's cells: read codes 8q .. 8q+7 against window
-            // codes from i + dlo + 8q (exact for rows and columns >= 1, the only
-            // cells whose characters can decide)
-            uint32_t rc;
-            if (rwn) {   // code rbase + 8q sits at place o of the staged codes
-                const int o = (int)((rbase - 1) & 15) + 1 + 8 * q;
+        const int g = (col - 1) >> 5, j0 = (col - 1) & 31;
+        if (i != ci || g != cg) {
+            // D words of blocks 4g .. 4g+3 (clamped to the read's last block: bits of
+            // columns past the walk's cell are never looked at)
+            int wd = 0, sh = 0;
+            dloc<NW>(imin(imax(i, 0), BMAX - 1), wd, sh);
+            uint32_t dw[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                dw[u] = scr[((size_t)imin(4 * g + u, qlast) * WB + wd) * 64 + lane];
+            // characters: read codes 32g .. 32g+31 against window codes from
+            // i + dlo + 32g (exact for rows and columns >= 1, the only cells whose
+            // characters can decide)
+            const int64_t wb = woff + (i + dlo + 32 * g);
+            const uint32_t *__restrict__ ww = a.win_words + (wb >> 4);
+            const uint32_t g0 = ww[0], g1 = ww[1], g2 = ww[2];
+            const uint32_t gsh = (uint32_t)(wb & 15) * 2u;
+            uint32_t r0, r1, r2, rsh;
+            if (rwn) {   // code rbase + 32g sits at place o of the staged codes
+                const int o = (int)((rbase - 1) & 15) + 1 + 32 * g;
                 const int wi = o >> 4;
-                rc = __builtin_amdgcn_alignbit(lcodes[(wi + 1) * 64], lcodes[wi * 64],
-                                               (uint32_t)(o & 15) * 2u);
+                r0 = lcodes[wi * 64];
+                r1 = lcodes[(wi + 1) * 64];
+                r2 = lcodes[(wi + 2) * 64];
+                rsh = (uint32_t)(o & 15) * 2u;
             } else {
-                rc = fetch16(a.read_words, rbase + 8 * q);
+                const int64_t rb = rbase + 32 * g;
+                const uint32_t *__restrict__ rw = a.read_words + (rb >> 4);
+                r0 = rw[0];
+                r1 = rw[1];
+                r2 = rw[2];
+                rsh = (uint32_t)(rb & 15) * 2u;
             }
-            const uint32_t gc = fetch16(a.win_words, woff + (i + dlo + 8 * q));
-            const uint32_t x = rc ^ gc;
-            uint32_t m8 = (x | (x >> 1)) & 0x5555u;
-            m8 = (m8 | (m8 >> 1)) & 0x3333u;
-            m8 = (m8 | (m8 >> 2)) & 0x0F0Fu;
-            mis = (m8 | (m8 >> 4)) & 0xFFu;
+            const uint32_t x0 = __builtin_amdgcn_alignbit(r1, r0, rsh) ^
+                                __builtin_amdgcn_alignbit(g1, g0, gsh);
+            const uint32_t x1 = __builtin_amdgcn_alignbit(r2, r1, rsh) ^
+                                __builtin_amdgcn_alignbit(g2, g1, gsh);
+            mseg = even_bits(x0 | (x0 >> 1)) | (even_bits(x1 | (x1 >> 1)) << 16);
+            dseg = 0u;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dseg |= ((dw[u] >> sh) & 0xFFu) << (8 * u);
+            if (i < 0 || i >= BMAX) dseg = 0u;
             ci = i;
-            cq = q;
-            side = false;
+            cg = g;
         }
-        // cells j0, j0-1, .. of the block step diagonally until one has D0 set
+        // cells j0, j0-1, .. of the segment step diagonally until one has D0 set
         // and differing characters
-        const uint32_t below = (2u << j0) - 1u;
-        const uint32_t fail = db & mis & below;
+        const uint32_t below = (2u << j0) - 1u;   // bits 0 .. j0 (j0 = 31: all)
+        const uint32_t fail = dseg & mseg & below;
         int rr = fail ? j0 - (31 - __builtin_clz(fail)) : j0 + 1;
         rr = imin(rr, imin(row, cap - n));
         if (rr > 0) {
             // rr diagonal steps: 0 bits in the transcript; v drops by the mismatches
             const uint32_t run = below & ~((1u << (j0 + 1 - rr)) - 1u);
-            v -= __builtin_popcount(~db & run);
-            if (((n & 15) + rr) >= 16) {
+            v -= __builtin_popcount(~dseg & run);
+            const int n2 = n + rr;
+            if ((n2 >> 4) != (n >> 4)) {   // the run completes the current ops word
                 ops[n >> 4] = opw;
                 opw = 0;
+                if ((n2 >> 4) > (n >> 4) + 1) ops[(n >> 4) + 1] = 0u;   // and a whole one
             }
-            n += rr;
+            n = n2;
             row -= rr;
             col -= rr;
             continue;
         }
         // one cell that is not on a diagonal run (col >= 1)
-        if (!side) {
+        const int q = (col - 1) >> 3, jb = (col - 1) & 7;
+        if (i != si || q != sq) {
+            const uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
             dm = dbyte(blk, i - 1);
             dp = dbyte(blk, i + 1);
             bvt = (int)blk[DW * 64];
@@ -597,12 +627,13 @@ void bits_trace_kernel(BandArgs a)
                 bvp[m] = blk[(DW + 1 + m) * 64];
                 bvn[m] = blk[(DW + 1 + NWD + m) * 64];
             }
-            side = true;
+            si = i;
+            sq = q;
         }
-        const int Dg = row >= 1 ? v - 1 + (int)((db >> j0) & 1u) : SRM_FOREIGN;
-        const int Tp = row >= 1 ? cell(i - 1, dm, j0 + 1) : SRM_FOREIGN;
-        const int Lf = cell(i + 1, dp, j0);
-        const bool eq = ((mis >> j0) & 1u) == 0u;   // decides only with row >= 1
+        const int Dg = row >= 1 ? v - 1 + (int)((dseg >> j0) & 1u) : SRM_FOREIGN;
+        const int Tp = row >= 1 ? cell(i - 1, dm, jb + 1) : SRM_FOREIGN;
+        const int Lf = cell(i + 1, dp, jb);
+        const bool eq = ((mseg >> j0) & 1u) == 0u;   // decides only with row >= 1
         const int op = decide(0, 1, v, Dg, Lf, Tp, col - 1, imax(row - 1, 0), L, w, s, e, eq);
         if (op == 3 || n >= cap) break;
         v = op == 0 ? Dg : op == 1 ? Tp : Lf;
