@@ -155,8 +155,8 @@ void bits_band_kernel(BandArgs a)
         td.z = __builtin_amdgcn_readfirstlane(td.z);
         uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
         const bool active = (uint32_t)lane < td.y;
-        const uint32_t t = active ? a.perm[td.x + lane] : 0u;
-        const srm_cand cd = a.cand[t];
+        const uint32_t pos = td.x + (active ? (uint32_t)lane : 0u);   // tile order
+        const srm_cand cd = a.cand_tile[pos];
         const int L = active ? (int)cd.L : 0;
         const int w = cd.w, k = cd.k;
         const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
@@ -405,7 +405,8 @@ void bits_band_kernel(BandArgs a)
             r.n_ops = 0;
             r.end_row = (uint16_t)w;
             r.end_col = (uint16_t)L;
-            a.res[t] = r;
+            a.res[a.perm[pos]] = r;
+            a.tile_score[pos] = score;
         }
     }
 }
@@ -424,11 +425,26 @@ void bits_trace_kernel(BandArgs a)
     if (tile >= a.ntiles) return;
     const uint4 td = a.tiles[tile];
     if ((uint32_t)lane >= td.y) return;
-    const uint32_t t = a.perm[td.x + lane];
-    const srm_cand cd = a.cand[t];
-    srm_result r = a.res[t];
+    const uint32_t pos = td.x + lane;   // tile order
+    const srm_cand cd = a.cand_tile[pos];
+    const int score = a.tile_score[pos];
+    // the packed tail (the first run's columns L - 1 and L), loaded with the
+    // descriptor rather than after the score test
+    const uint32_t *__restrict__ tail =
+        a.scratch + a.tile_scratch[tile] + (size_t)((td.z + 8) >> 3) * BMAX * 64 + lane;
+    typedef uint32_t TailWords __attribute__((ext_vector_type(NW)));
+    TailWords tl, tp;
+#pragma unroll
+    for (int m = 0; m < NW; ++m) {
+        tl[m] = tail[m * 64];
+        tp[m] = tail[(NW + m) * 64];
+    }
+    const uint32_t eq0 = tail[(2 * NW) * 64], eq1 = tail[(2 * NW + 1) * 64];
     const int k = cd.k, K = k + 1;
-    if (r.score > k) return;
+    if (score > k) return;
+    const uint32_t t = a.perm[pos];
+    srm_result r;
+    r.score = (int16_t)score;
     const int L = cd.L, w = cd.w;
     const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
     const int dlo = -k;
@@ -453,16 +469,6 @@ void bits_trace_kernel(BandArgs a)
     };
     {
         // the first run, up column L (its free tail rows), from the packed tail
-        const uint32_t *__restrict__ tail =
-            scr + (size_t)((td.z + 8) >> 3) * BMAX * 64 + lane;
-        typedef uint32_t TailWords __attribute__((ext_vector_type(NW)));
-        TailWords tl, tp;
-#pragma unroll
-        for (int m = 0; m < NW; ++m) {
-            tl[m] = tail[m * 64];
-            tp[m] = tail[(NW + m) * 64];
-        }
-        const uint32_t eq0 = tail[(2 * NW) * 64], eq1 = tail[(2 * NW + 1) * 64];
         auto nib = [&](const TailWords &X, int i) -> int {
             if (i < 0 || i >= BMAX) return 15;
             return (int)((X[i >> 3] >> ((i & 7) * 4)) & 15u);

@@ -35,6 +35,8 @@ struct BandArgs {
     int ma, mm;
     const srm_cand *cand;
     const uint32_t *perm;           // candidate indices, tile-major
+    const srm_cand *cand_tile;      // the candidates in the same order (unit-cost kernels)
+    int32_t *tile_score;            // scores in the same order (bits_band_kernel -> walk)
     const uint4 *tiles;             // x perm start, y count (<= 64), z max L, w max k
     uint32_t ntiles;
     uint32_t *queue;                // tile counter (zeroed before the launch)

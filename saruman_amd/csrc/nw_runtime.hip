@@ -145,7 +145,7 @@ struct srm_job {
     bool resident_genome = false;
     // device buffers
     DevBuf d_cand, d_read_words, d_win_words, d_read_bytes, d_win_bytes;
-    DevBuf d_perm, d_tiles, d_tile_scratch, d_queue, d_scratch;
+    DevBuf d_perm, d_cand_tile, d_tile_score, d_tiles, d_tile_scratch, d_queue, d_scratch;
     DevBuf d_idx_rest, d_idx_all, d_idx_bytes;      // plain index lists
     DevBuf d_off_rest, d_off_all, d_off_bytes;      // matrix offsets (per list, chunk-relative)
     DevBuf d_mat, d_res, d_ops;
@@ -270,6 +270,8 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.mm = j->mm;
             a.cand = (const srm_cand *)j->d_cand.p;
             a.perm = (const uint32_t *)j->d_perm.p;
+            a.cand_tile = (const srm_cand *)j->d_cand_tile.p;
+            a.tile_score = (int32_t *)j->d_tile_score.p;
             a.tiles = (const uint4 *)j->d_tiles.p + bc.tile_begin;
             a.ntiles = bc.ntiles;
             a.queue = (uint32_t *)j->d_queue.p + c;
@@ -503,7 +505,8 @@ void srm_job_destroy(srm_job *j)
     (void)hipStreamSynchronize(c->stream);
     destroy_graph(j);
     DevBuf *bufs[] = {&j->d_cand, &j->d_read_words, &j->d_win_words, &j->d_read_bytes,
-                      &j->d_win_bytes, &j->d_perm, &j->d_tiles, &j->d_tile_scratch,
+                      &j->d_win_bytes, &j->d_perm, &j->d_cand_tile, &j->d_tile_score,
+                      &j->d_tiles, &j->d_tile_scratch,
                       &j->d_queue, &j->d_scratch,
                       &j->d_idx_rest, &j->d_idx_all, &j->d_idx_bytes, &j->d_off_rest,
                       &j->d_off_all, &j->d_off_bytes, &j->d_mat, &j->d_res, &j->d_ops,
@@ -725,6 +728,15 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     if ((rc = upload(ctx, j->d_read_bytes, read_bytes, read_bytes ? n_read_bytes : 0))) return rc;
     if ((rc = upload(ctx, j->d_win_bytes, win_bytes, win_bytes ? n_win_bytes : 0))) return rc;
     if ((rc = upload(ctx, j->d_perm, band_idx.data(), band_idx.size()))) return rc;
+    {
+        // the banded candidates in tile order: a wave reads its tile's descriptors
+        // as one contiguous run, without the indirection through perm
+        std::vector<srm_cand> ct(band_idx.size());
+        for (size_t p = 0; p < band_idx.size(); ++p) ct[p] = cand[band_idx[p]];
+        if ((rc = upload(ctx, j->d_cand_tile, ct.data(), ct.size()))) return rc;
+        if ((rc = ctx->ensure(j->d_tile_score, std::max<size_t>(band_idx.size() * 4, 64))))
+            return rc;
+    }
     if ((rc = upload(ctx, j->d_tiles, tiles.data(), tiles.size()))) return rc;
     if ((rc = upload(ctx, j->d_tile_scratch, tile_off.data(), tile_off.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_rest, rest_idx.data(), rest_idx.size()))) return rc;
