@@ -41,11 +41,14 @@ PMC_TRAFFIC = [
 
 
 def kernel_sha() -> str:
+    """sha256 prefix of the alignment kernels' sources (both band kernel files and
+    their shared helpers)."""
     import hashlib
-    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "saruman_amd", "csrc",
-                       "nw_kernels.hip")
-    with open(src, "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()[:12]
+    h = hashlib.sha256()
+    for name in ("band_bits.hip", "nw_kernels.hip", "nw_seq.h"):
+        with open(os.path.join(ROOT, "saruman_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:12]
 
 
 def tuning_key(tuning):
@@ -393,8 +396,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS,
                      "traffic": profiled_traffic(args.workload, int(db.n), tuning),
-                     "kernel": "band_kernel + trace_kernel of every band class of one step "
-                               "(HIP events around the band phase)",
+                     "kernel": "bits_band_kernel + bits_trace_kernel of every band class of one "
+                               "step (HIP events around the band phase)",
                      "ops_per_step": OPS_PER_BAND_CELL * bcells,
                      "kernel_ms": band_ms},
         "roofline_hbm": {"achieved": hbm / (band_ms / 1e3) / 1e9, "peak": PEAK_HBM_GBS,

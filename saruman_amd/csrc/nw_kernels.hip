@@ -1,18 +1,20 @@
 // nw_kernels.hip -- gfx950 kernels for SARUMAN's semiglobal Needleman-Wunsch with
 // traceback (the work of compute_score/get_alignment, src/cuda-functions.cu:76-536).
 //
-// Two kernels compute the same results:
+// The kernels here compute the same results:
 //   plain_kernel  one candidate per thread, full (w+1)x(L+1) int16 matrix in HBM,
 //                 value-based traceback on that matrix.  Handles every candidate
 //                 (any match/mismatch, raw-byte or 2-bit sequences).
 //   band_kernel   persistent: each wave pulls 64-candidate tiles from a device queue;
-//                 one candidate per lane; the exact k-error band of a DP column
+//   trace_kernel  one candidate per lane; the exact k-error band of a DP column
 //                 (d = row - col in [-k, max(2s+k, w-L)]) lives in VGPRs, saturated
 //                 at k+1; each column is stored as nibbles in a per-wave scratch
 //                 ([column][word][lane], coalesced 256-byte stores) and the traceback
-//                 walks those nibbles.  Needs match >= 0, mismatch >= 1, k <= 14.
+//                 walks those nibbles.  Needs match >= 0, mismatch >= 1, k <= 14;
+//                 serves the costs other than match 0 / mismatch 1 (unit costs run
+//                 the bit-parallel kernels of band_bits.hip).
 //
-// Both reproduce the reference recurrence and traceback decisions bit for bit
+// All reproduce the reference recurrence and traceback decisions bit for bit
 // (quirks Q4-Q7 kept, Q1 selectable per candidate, Q2/Q3 canonical).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -149,7 +151,7 @@ struct BandWaves {
         NW <= 1 ? 8 : NW <= 2 ? 7 : NW <= 3 ? 6 : NW <= 5 ? 4 : NW <= 7 ? 3 : 2;
 };
 
-template <int NW, bool UNIT>
+template <int NW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BandWaves<NW>::value)))
 void band_kernel(BandArgs a)
 {
@@ -181,21 +183,15 @@ void band_kernel(BandArgs a)
         const int dlo = -k;
         const int K = k + 1;
         const int Lmax = (int)td.z;
-        // unit costs: a band value grows by at most 1 per column, so saturating
-        // every (15 - max k) columns keeps every stored value <= 15
-        const int satp = UNIT ? 15 - (int)td.w : 1;
 
         // ---- column 0: M[h][0] = max(0, h - 2s) (unit steps, Q7); rows < 0 -> K
-        // Unit costs run in the potential W = M - 2c - i (column c, band cell i): the
-        // left and top moves then cost 0 and the diagonal costs bit - 2, so a cell is
-        // min3(W + bit - 2, W_left, W_up).  Other costs run on M itself.
         int v[BMAX];
         uint32_t acc[BMAX];
 #pragma unroll
         for (int i = 0; i < BMAX; ++i) {
             const int h = dlo + i;
             const int m0 = h < 0 ? K : imin(imax(h - 2 * s, 0), K);
-            v[i] = UNIT ? m0 - i : m0;
+            v[i] = m0;
             acc[i] = (uint32_t)v[i];
         }
         // window codes for column 1: W[i] = g[dlo + i]; a 16*WW-code window plus a stream
@@ -206,11 +202,10 @@ void band_kernel(BandArgs a)
         uint32_t gnext = 0;   // stream of window codes entering the top of the window
         uint32_t rbuf = 0;    // stream of read codes
         const int64_t rbase = (int64_t)cd.read_off;
-        int since_sat = 0;
 
         // one DP column; last selects the free trailing verticals of column L
         uint32_t lastmis[WW];   // mismatch fields of column L
-        auto column = [&](int c, bool last, bool satc) {
+        auto column = [&](int c, bool last) {
             // window for column c (c >= 2): shift by one code, take the next from the stream
             if (c >= 2) {
                 if (((c - 2) & 15) == 0) gnext = fetch16(a.win_words, gbase + 16 * WW + (c - 2));
@@ -237,82 +232,35 @@ void band_kernel(BandArgs a)
             }
             int ifree = BMAX;            // first band cell whose top move is free
             if (last) ifree = w - 2 * e + 1 - c - dlo;
-            const bool sat = !UNIT || last || satc;
             const uint32_t sh = (uint32_t)(c & 7) * 4u;
-            if (UNIT) {
-                uint32_t misf[WW];
+            int up = 1 << 20;
 #pragma unroll
-                for (int j = 0; j < WW; ++j) misf[j] = mis[j] | 0xAAAAAAAAu;
-                int up = 1 << 28;
-                const int satv = K - 2 * c;   // W bound of cell i: K - 2c - i
-#pragma unroll
-                for (int i = 0; i < BMAX; ++i) {
-                    // field i of misf is 0b10 (match) or 0b11 (mismatch): read signed it
-                    // is the diagonal cost in W, -2 or -1, in one v_bfe_i32
-                    const int dcost = __builtin_amdgcn_sbfe((int)misf[i >> 4], 2 * (i & 15), 2);
-                    const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 28);
-                    // the top move into a row of the free tail of column L costs 0: -1 in W
-                    const int upc = (last && i >= ifree) ? up - 1 : up;
-                    int nv = imin(imin(v[i] + dcost, lf), upc);
-                    if (last) nv = imin(nv, satv - i);
-                    v[i] = nv;
-                    up = nv;
-                }
-                if (sat && !last) {
-                    // the periodic clip uses the tile's bound (max k + 1 >= K, the same
-                    // on every lane: the bounds are scalar): values <= k stay exact,
-                    // larger ones stay above K, and the saturation period 15 - max k
-                    // keeps every stored value <= 15
-                    const int satu = (int)td.w + 1 - 2 * c;
-#pragma unroll
-                    for (int i = 0; i < BMAX; ++i) v[i] = imin(v[i], satu - i);
-                }
-            } else {
-                int up = 1 << 20;
-#pragma unroll
-                for (int i = 0; i < BMAX; ++i) {
-                    const int bit = (int)((mis[i >> 4] >> (2 * (i & 15))) & 1u);
-                    const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 20);
-                    const int dg = v[i] + (bit ? mm : ma);
-                    const int tcost = (last && i >= ifree) ? 0 : mm;
-                    int nv = imin(imin(dg, lf + mm), up + tcost);
-                    nv = imin(nv, K);
-                    v[i] = nv;
-                    up = nv;
-                }
-                if (c <= k) {
-                    // row 0 keeps its unit-step init M[0][c] = c (Q7) whatever the mismatch cost
-                    const int i0 = k - c;
-#pragma unroll
-                    for (int i = 0; i < BMAX; ++i) v[i] = (i == i0) ? imin(c, K) : v[i];
-                }
+            for (int i = 0; i < BMAX; ++i) {
+                const int bit = (int)((mis[i >> 4] >> (2 * (i & 15))) & 1u);
+                const int lf = (i + 1 < BMAX) ? v[i + 1] : (1 << 20);
+                const int dg = v[i] + (bit ? mm : ma);
+                const int tcost = (last && i >= ifree) ? 0 : mm;
+                int nv = imin(imin(dg, lf + mm), up + tcost);
+                nv = imin(nv, K);
+                v[i] = nv;
+                up = nv;
             }
-            // nibble c & 7 of every diagonal's word (an add: unit-cost values are
-            // potentials, which may be negative; the block's sum is exact mod 2^32);
-            // a block is stored when complete and its words start again from 0
+            if (c <= k) {
+                // row 0 keeps its unit-step init M[0][c] = c (Q7) whatever the mismatch cost
+                const int i0 = k - c;
+#pragma unroll
+                for (int i = 0; i < BMAX; ++i) v[i] = (i == i0) ? imin(c, K) : v[i];
+            }
+            // nibble c & 7 of every diagonal's word; a block is stored when complete
+            // and its words start again from 0
 #pragma unroll
             for (int i = 0; i < BMAX; ++i) acc[i] += (uint32_t)v[i] << sh;
             if ((c & 7) == 7 || last) {
                 // streaming stores: the band is read back only by the walk, much
                 // later; the code words the next columns fetch stay in L2
                 uint32_t *__restrict__ blk = scr + (size_t)(c >> 3) * BMAX * 64 + lane;
-                if (UNIT) {
-                    // back from W: nibble j of word (q, i) gets 2(8q + j) + i, i.e. the
-                    // word gets (16q + i) * sum_j 16^j + 2 * sum_j j 16^j over the
-                    // nibbles written so far (exact modulo 2^32)
-                    const uint32_t msk = (c & 7) == 7 ? 0xFFFFFFFFu : ((4u << sh) << 2) - 1u;
-                    const uint32_t ones = 0x11111111u & msk;
-                    const uint32_t base = 0xECA86420u & msk;
-                    const uint32_t q16 = (uint32_t)(c >> 3) * 16u;
 #pragma unroll
-                    for (int i = 0; i < BMAX; ++i)
-                        __builtin_nontemporal_store(acc[i] + (q16 + (uint32_t)i) * ones + base,
-                                                    &blk[i * 64]);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < BMAX; ++i)
-                        __builtin_nontemporal_store(acc[i], &blk[i * 64]);
-                }
+                for (int i = 0; i < BMAX; ++i) __builtin_nontemporal_store(acc[i], &blk[i * 64]);
 #pragma unroll
                 for (int i = 0; i < BMAX; ++i) acc[i] = 0u;
             }
@@ -322,9 +270,7 @@ void band_kernel(BandArgs a)
             // the column is the same on every lane: kept scalar, so the steps taken
             // every 8 and 16 columns are scalar branches, not masked ones
             const int c = __builtin_amdgcn_readfirstlane(c0);
-            const bool satc = ++since_sat >= satp;   // every satp columns, on every lane
-            if (satc) since_sat = 0;
-            if (c < L) column(c, false, satc);
+            if (c < L) column(c, false);
         }
         // tail of the tile's scratch, for the walk's first (vertical) run: columns
         // L and L - 1 by diagonal (nibble j of word m is diagonal 8m + j), then the
@@ -337,15 +283,14 @@ void band_kernel(BandArgs a)
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const int i = 8 * m + j;
-                    const int mv = UNIT ? v[i] + 2 * c + i : v[i];
-                    x |= ((uint32_t)mv & 15u) << (4 * j);
+                    x |= ((uint32_t)v[i] & 15u) << (4 * j);
                 }
                 __builtin_nontemporal_store(x, &tail[(word0 + m) * 64]);
             }
         };
         if (active) {
             store_column(L - 1, NW);
-            column(L, true, true);
+            column(L, true);
             store_column(L, 0);
             uint32_t eqw[2] = {0u, 0u};
 #pragma unroll
@@ -381,19 +326,6 @@ __device__ __forceinline__ uint32_t band_word_or_k(const uint32_t *__restrict__ 
     return (i >= 0 && i < BMAX && q >= 0) ? band_word<BMAX>(scr, i, q, lane) : 0xFFFFFFFFu;
 }
 
-// nibble-parallel helpers of the walk: 8 nibbles of a band word at once
-// per-nibble (x - 1) mod 16
-__device__ __forceinline__ uint32_t nib_dec(uint32_t x)
-{
-    return ((x | 0x88888888u) - 0x11111111u) ^ (~x & 0x88888888u);
-}
-// bit 4j set <=> nibble j of x is nonzero
-__device__ __forceinline__ uint32_t nib_nz(uint32_t x)
-{
-    x |= x >> 1;
-    x |= x >> 2;
-    return x & 0x11111111u;
-}
 // 8 two-bit fields (bits 2j..2j+1) moved to nibbles (bits 4j..4j+1)
 __device__ __forceinline__ uint32_t spread2(uint32_t f)
 {
@@ -410,7 +342,7 @@ __device__ __forceinline__ uint32_t spread2(uint32_t f)
 // a block of 8 columns is found from the block's word at once, the other steps
 // (gaps, the free rows of column L, the start of the read) one cell at a time;
 // column 0 comes from its closed form max(0, h - 2s) (Q7) without loads.
-template <int NW, bool UNIT>
+template <int NW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 void trace_kernel(BandArgs a)
 {
@@ -556,31 +488,17 @@ void trace_kernel(BandArgs a)
             side = false;
         }
         const int j0 = col & 7;
+        // the first two rules of decide(), one cell at a time
         int rr;
-        if (UNIT) {
-            // unit costs: cell j steps diagonally iff v_j == D_j + 1 (mismatch cost)
-            // or v_j == D_j with equal characters, D_j the diagonal predecessor's
-            // stored nibble (nibble j-1, or P0's nibble 7 for j = 0).  On the walk
-            // v <= k < K, and a successful step keeps the next cell <= k, so the
-            // clip of D at K never decides a cell of the run.
-            const uint32_t N = __builtin_amdgcn_alignbit(W0, P0, 28);
-            const uint32_t fail_mm = nib_nz(nib_dec(W0) ^ N) | (~nib_nz(W0) & 0x11111111u);
-            const uint32_t fail_ma = nib_nz(W0 ^ N) | mis;
-            const uint32_t fail = fail_mm & fail_ma & ((0x2u << (4 * j0)) - 1u);
-            rr = fail ? j0 - ((31 - __builtin_clz(fail)) >> 2) : j0 + 1;
-            rr = imin(imin(rr, col), imin(row, cap - n));
-        } else {
-            // other costs: the first two rules of decide() one cell at a time
-            const int rmax = imin(j0 + 1, imin(imin(col, row), cap - n));
-            rr = 0;
-            while (rr < rmax) {
-                const int j2 = j0 - rr;
-                const int v2 = (int)((W0 >> (4 * j2)) & 15u);
-                const int d2 =
-                    imin(j2 ? (int)((W0 >> (4 * j2 - 4)) & 15u) : (int)(P0 >> 28), K);
-                if (d2 + mm != v2 && (d2 + ma != v2 || ((mis >> (4 * j2)) & 1u))) break;
-                ++rr;
-            }
+        const int rmax = imin(j0 + 1, imin(imin(col, row), cap - n));
+        rr = 0;
+        while (rr < rmax) {
+            const int j2 = j0 - rr;
+            const int v2 = (int)((W0 >> (4 * j2)) & 15u);
+            const int d2 =
+                imin(j2 ? (int)((W0 >> (4 * j2 - 4)) & 15u) : (int)(P0 >> 28), K);
+            if (d2 + mm != v2 && (d2 + ma != v2 || ((mis >> (4 * j2)) & 1u))) break;
+            ++rr;
         }
         if (rr > 0) {
             // rr diagonal steps: 0 bits in the transcript
@@ -667,7 +585,7 @@ hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st)
 template <int NW>
 static hipError_t launch_band_nw(const BandArgs &a, int blocks, hipStream_t st)
 {
-    hipLaunchKernelGGL((band_kernel<NW, false>), dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((band_kernel<NW>), dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -676,7 +594,7 @@ static int occupancy_nw()
 {
     int blocks = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocks, (const void *)band_kernel<NW, false>, 256, 0);
+        &blocks, (const void *)band_kernel<NW>, 256, 0);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
@@ -717,7 +635,7 @@ hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStre
 template <int NW>
 static hipError_t launch_trace_nw(const BandArgs &a, size_t lds, hipStream_t st)
 {
-    hipLaunchKernelGGL((trace_kernel<NW, false>), dim3((a.ntiles + 3) / 4), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((trace_kernel<NW>), dim3((a.ntiles + 3) / 4), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
