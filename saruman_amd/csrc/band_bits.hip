@@ -197,17 +197,34 @@ void bits_band_kernel(BandArgs a)
             ch1[m] = 0u;
             if (active) planes32(a.win_words, gbase + 32 * m, ch0[m], ch1[m]);
         }
-        uint32_t rbuf = 0;
-        // the next refills, loaded one block ahead (before the block's stores, so
-        // waiting for them never waits for the stores): raw read code words for
-        // the next 16 columns, raw window code words for the next chunk
-        const uint32_t rsh0 = (uint32_t)(rbase & 15) * 2u;   // code offset of rbase + 16m
-        const uint32_t gsh0 = (uint32_t)(gbase & 15) * 2u;   // and of gbase + 32m
-        uint32_t rr0 = 0, rr1 = 0, pg0 = 0, pg1 = 0, pg2 = 0;
-        if (active) {
-            rr0 = a.read_words[rbase >> 4];
-            rr1 = a.read_words[(rbase >> 4) + 1];
+        // the read's codes, staged in this wave's LDS for reads of up to 250 codes
+        // (lane-interleaved words; word u holds read_words word ((rbase - 1) >> 4) + u):
+        // one pass over the read instead of a reload every 16 columns
+        extern __shared__ uint32_t band_lds[];
+        const int rwn = a.code_words;
+        uint32_t *__restrict__ lcodes = band_lds + (size_t)(threadIdx.x >> 6) * 64 * rwn + lane;
+        if (rwn && active) {
+            const uint32_t *__restrict__ src = a.read_words + ((rbase - 1) >> 4);
+            const int nwd = ((L + 15) >> 4) + 3;   // <= rwn: L <= the class's max L
+            for (int u = 0; u < nwd; ++u) lcodes[u * 64] = src[u];
         }
+        // 16 read codes from code offset o (staged: code rbase + o is at place
+        // rp0 + o; every place read is inside the lane's rwn words)
+        const int rp0 = (int)((rbase - 1) & 15) + 1;
+        auto read16 = [&](int o) -> uint32_t {
+            if (rwn) {
+                const int p = rp0 + o;
+                const int wi = p >> 4;
+                return __builtin_amdgcn_alignbit(lcodes[(wi + 1) * 64], lcodes[wi * 64],
+                                                 (uint32_t)(p & 15) * 2u);
+            }
+            return fetch16(a.read_words, rbase + o);
+        };
+        uint32_t rbuf = 0;
+        // the next chunk of window codes, loaded one block ahead (before the block's
+        // stores, so waiting for them never waits for the stores)
+        const uint32_t gsh0 = (uint32_t)(gbase & 15) * 2u;   // code offset of gbase + 32m
+        uint32_t pg0 = 0, pg1 = 0, pg2 = 0;
         uint32_t d[8][NWD];   // D0 words of the group's 8 columns
 #pragma unroll
         for (int j = 0; j < 8; ++j)
@@ -229,7 +246,7 @@ void bits_band_kernel(BandArgs a)
                 planes_of(__builtin_amdgcn_alignbit(pg1, pg0, gsh0),
                           __builtin_amdgcn_alignbit(pg2, pg1, gsh0), ch0[NWD], ch1[NWD]);
             }
-            if (((cb - 1) & 15) == 0) rbuf = __builtin_amdgcn_alignbit(rr1, rr0, rsh0);
+            if (((cb - 1) & 15) == 0) rbuf = read16(cb - 1);
             store_boundary(q);   // column cb - 1 = 8q (garbage on lanes past their read)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -300,16 +317,11 @@ void bits_band_kernel(BandArgs a)
                     }
                 }
             }
-            // loads for the next block's refills (columns from cb + 8), issued before
-            // this block's stores
+            // loads for the next block's window refill (columns from cb + 8), issued
+            // before this block's stores
             {
                 const int o2 = cb + 7;
                 if (cb + 8 < L) {
-                    if ((o2 & 15) == 0) {
-                        const int64_t wi = (rbase + o2) >> 4;
-                        rr0 = a.read_words[wi];
-                        rr1 = a.read_words[wi + 1];
-                    }
                     if ((o2 & 31) == 0) {
                         const int64_t wi = (gbase + o2 + 32 * NWD) >> 4;
                         pg0 = a.win_words[wi];
@@ -363,7 +375,7 @@ void bits_band_kernel(BandArgs a)
             uint32_t gw[WW];
 #pragma unroll
             for (int u = 0; u < WW; ++u) gw[u] = fetch16(a.win_words, gbase + (L - 1) + 16 * u);
-            const uint32_t rc = code_at(a.read_words, rbase + L - 1);
+            const uint32_t rc = read16(L - 1) & 3u;
             const uint32_t rrep = rc * 0x55555555u;
             const int ifree = w - 2 * e + 1 - L - dlo;   // first cell with a free top move
             const int iend = w - L - dlo;                // cell of row w
@@ -656,23 +668,28 @@ void bits_trace_kernel(BandArgs a)
 // host-side launch wrappers
 // ---------------------------------------------------------------------------
 template <int NW>
-static hipError_t launch_bits_band_nw(const BandArgs &a, int blocks, hipStream_t st)
+static hipError_t launch_bits_band_nw(const BandArgs &a, int blocks, size_t lds, hipStream_t st)
 {
-    hipLaunchKernelGGL(bits_band_kernel<NW>, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(bits_band_kernel<NW>, dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, hipStream_t st)
+// reads of up to 250 codes stage their codes in LDS (19 KiB per block of 4 waves,
+// 8 blocks per CU); longer ones are read from read_words
+hipError_t launch_bits_band(const BandArgs &args, int nw, int blocks, int maxL, hipStream_t st)
 {
+    BandArgs a = args;
+    a.code_words = maxL <= 250 ? ((maxL + 15) >> 4) + 3 : 0;
+    const size_t lds = (size_t)a.code_words * 256 * 4;
     switch (nw) {
-    case 1: return launch_bits_band_nw<1>(a, blocks, st);
-    case 2: return launch_bits_band_nw<2>(a, blocks, st);
-    case 3: return launch_bits_band_nw<3>(a, blocks, st);
-    case 4: return launch_bits_band_nw<4>(a, blocks, st);
-    case 5: return launch_bits_band_nw<5>(a, blocks, st);
-    case 6: return launch_bits_band_nw<6>(a, blocks, st);
-    case 7: return launch_bits_band_nw<7>(a, blocks, st);
-    case 8: return launch_bits_band_nw<8>(a, blocks, st);
+    case 1: return launch_bits_band_nw<1>(a, blocks, lds, st);
+    case 2: return launch_bits_band_nw<2>(a, blocks, lds, st);
+    case 3: return launch_bits_band_nw<3>(a, blocks, lds, st);
+    case 4: return launch_bits_band_nw<4>(a, blocks, lds, st);
+    case 5: return launch_bits_band_nw<5>(a, blocks, lds, st);
+    case 6: return launch_bits_band_nw<6>(a, blocks, lds, st);
+    case 7: return launch_bits_band_nw<7>(a, blocks, lds, st);
+    case 8: return launch_bits_band_nw<8>(a, blocks, lds, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -77,10 +77,10 @@ hipError_t launch_rec_measure(const RecArgs &a, hipStream_t st);
 hipError_t launch_rec_write(const RecArgs &a, hipStream_t st);
 hipError_t rec_scan(void *tmp, size_t &tmp_bytes, const uint64_t *len, uint64_t *off, int n,
                     hipStream_t st);
-hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st);
+hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, int maxL, hipStream_t st);
 hipError_t launch_trace(const BandArgs &a, int nw, bool unit, int maxL, hipStream_t st);
 // unit costs (band_bits.hip): the bit-parallel forward pass and its walk
-hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, hipStream_t st);
+hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, int maxL, hipStream_t st);
 hipError_t launch_bits_trace(const BandArgs &a, int nw, size_t lds, hipStream_t st);
 int bits_band_occupancy(int nw);
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);

@@ -616,9 +616,9 @@ int band_occupancy(int nw, bool unit)
     }
 }
 
-hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, hipStream_t st)
+hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, int maxL, hipStream_t st)
 {
-    if (unit) return launch_bits_band(a, nw, blocks, st);
+    if (unit) return launch_bits_band(a, nw, blocks, maxL, st);
     switch (nw) {
     case 1: return launch_band_nw<1>(a, blocks, st);
     case 2: return launch_band_nw<2>(a, blocks, st);

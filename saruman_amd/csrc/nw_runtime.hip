@@ -284,7 +284,7 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.res = (srm_result *)j->d_res.p;
             a.ops = (uint32_t *)j->d_ops.p;
             if (timed) HIP_TRY(hipEventRecord(j->cs[c], cst));
-            HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, cst));
+            HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, bc.maxL, cst));
             HIP_TRY(srm::launch_trace(a, bc.nw, j->unit, bc.maxL, cst));
             if (timed) HIP_TRY(hipEventRecord(j->ce[c], cst));
         }
