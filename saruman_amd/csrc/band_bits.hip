@@ -44,10 +44,11 @@ struct BitsShape {
     static constexpr int BMAX = 8 * NW;                  // band cells stored per column
     static constexpr int NWD = NW <= 4 ? 1 : 2;          // 32-bit words of a column vector
     // the transposed D bits: word j of a half holds diagonals j, 8+j, 16+j, 24+j
-    // (one byte each); halves with fewer diagonals are packed into fewer words
+    // (one byte each, n of them used); a half is stored as the stream of its 8n
+    // used bytes (byte n*j + g = diagonal 8g + j), 2n words
     static constexpr int NLO = NW <= 4 ? NW : 4;         // bytes used per word, diagonals 0-31
     static constexpr int NHI = NW <= 4 ? 0 : NW - 4;     // diagonals 32-63
-    static constexpr int dwords(int n) { return n == 0 ? 0 : n >= 3 ? 8 : n == 2 ? 4 : 2; }
+    static constexpr int dwords(int n) { return 2 * n; }
     static constexpr int DWLO = dwords(NLO);
     static constexpr int DW = DWLO + dwords(NHI);        // D words per block
     static constexpr int WB = DW + 1 + 2 * NWD;          // + cell 0's value, +1 deltas, -1 deltas
@@ -62,17 +63,9 @@ __device__ __forceinline__ void dloc(int i, int &word, int &shift)
     const int n = hi ? S::NHI : S::NLO;
     const int base = hi ? S::DWLO : 0;
     const int ii = hi ? i - 32 : i;
-    const int j = ii & 7, g = ii >> 3;
-    if (n >= 3) {
-        word = base + j;
-        shift = 8 * g;
-    } else if (n == 2) {
-        word = base + (j & 3);
-        shift = 8 * (2 * g + (j >> 2));
-    } else {
-        word = base + (j & 1);
-        shift = 8 * (j >> 1);
-    }
+    const int t = n * (ii & 7) + (ii >> 3);   // place in the half's byte stream
+    word = base + (t >> 2);
+    shift = 8 * (t & 3);
 }
 
 // bits 0, 2, .., 30 of x moved to bits 0 .. 15
@@ -224,6 +217,11 @@ void bits_band_kernel(BandArgs a)
         // the next chunk of window codes, loaded one block ahead (before the block's
         // stores, so waiting for them never waits for the stores)
         const uint32_t gsh0 = (uint32_t)(gbase & 15) * 2u;   // code offset of gbase + 32m
+        // column L's window codes, for the tail: loaded now, used after the loop
+        uint32_t gw[WW];
+#pragma unroll
+        for (int u = 0; u < WW; ++u)
+            gw[u] = active ? fetch16(a.win_words, gbase + (L - 1) + 16 * u) : 0u;
         uint32_t pg0 = 0, pg1 = 0, pg2 = 0;
         uint32_t d[8][NWD];   // D0 words of the group's 8 columns
 #pragma unroll
@@ -342,23 +340,16 @@ void bits_band_kernel(BandArgs a)
                 if (m == 0) diag0 = x[0] & 0xFFu;   // diagonal 0, columns cb .. cb+7
                 const int n = m == 0 ? S::NLO : S::NHI;
                 const int base = m == 0 ? 0 : S::DWLO;
-                if (n >= 3) {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) __builtin_nontemporal_store(x[j], &blk[(base + j) * 64]);
-                } else if (n == 2) {
+                for (int u = 0; u < 2 * n; ++u) {
+                    // bytes 4u .. 4u+3 of the stream (all indices are constants)
+                    uint32_t y = 0;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t y = (x[u] & 0xFFu) | ((x[u + 4] & 0xFFu) << 8) |
-                                           ((x[u] & 0xFF00u) << 8) | ((x[u + 4] & 0xFF00u) << 16);
-                        __builtin_nontemporal_store(y, &blk[(base + u) * 64]);
+                    for (int b = 0; b < 4; ++b) {
+                        const int t = 4 * u + b;
+                        y |= ((x[t / n] >> (8 * (t % n))) & 0xFFu) << (8 * b);
                     }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const uint32_t y = (x[u] & 0xFFu) | ((x[u + 2] & 0xFFu) << 8) |
-                                           ((x[u + 4] & 0xFFu) << 16) | ((x[u + 6] & 0xFFu) << 24);
-                        __builtin_nontemporal_store(y, &blk[(base + u) * 64]);
-                    }
+                    __builtin_nontemporal_store(y, &blk[(base + u) * 64]);
                 }
             }
             // band cell 0 grows by 1 per column where its D0 bit is 0
@@ -372,9 +363,6 @@ void bits_band_kernel(BandArgs a)
         // column L with the free top moves of its trailing rows.
         uint32_t *__restrict__ tail = scr + (size_t)((Lmax + 8) >> 3) * BMAX * 64 + lane;
         if (active) {
-            uint32_t gw[WW];
-#pragma unroll
-            for (int u = 0; u < WW; ++u) gw[u] = fetch16(a.win_words, gbase + (L - 1) + 16 * u);
             const uint32_t rc = read16(L - 1) & 3u;
             const uint32_t rrep = rc * 0x55555555u;
             const int ifree = w - 2 * e + 1 - L - dlo;   // first cell with a free top move
