@@ -29,14 +29,14 @@ PEAK_INT32_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz, int32 VALU ops/s
 PEAK_HBM_GBS = 8000.0
 OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 
-# HBM bytes of one step of the band phase (band_kernel and trace_kernel of every
+# HBM bytes of one step of the band phase (bits_band_kernel and bits_trace_kernel of every
 # class), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.000 and x1.000
 # by a 256 MiB copy kernel), for the exact configuration profiled: workload,
 # candidates, launch tuning and the sha256 prefix of the kernel source.
-# Source: profiles/band_and_trace_kernels_mixed_2M_reads.md
+# Source: profiles/bits_band_and_trace_kernels_mixed_2M_reads.md
 PMC_TRAFFIC = [
     {"workload": "mixed", "candidates": 1774238, "tuning": "defaults",
-     "kernel_sha": "e6fdbc86f66a", "bytes": 8993890778},
+     "kernel_sha": "4c507903b68b", "bytes": 4935084248},
 ]
 
 

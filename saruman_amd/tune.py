@@ -1,12 +1,14 @@
-"""Autotuner for the banded kernel's launch shape on this GPU.
+"""Autotuner for the banded kernels' launch shape on this GPU.
 
-For every band class (8*nw band cells per lane, nw = 1..8) it measures the
-persistent kernel with 1..max resident blocks per CU, each class timed with HIP
-events of its own launch, then tries running the narrow classes on a wider
-kernel (fewer launches, fewer tails) and running the classes concurrently on
-2 or 4 streams, timing whole steps.  The result is kept only if it beats the
-default launch by 2%.  The winner is cached in
-saruman_amd/tuning/<arch>.json and applied with srm_ctx_set_tuning().
+For costs other than match 0 / mismatch 1, and every band class (8*nw band
+cells per lane, nw = 1..8), it measures the persistent nibble kernel with
+1..max resident blocks per CU, each class timed with HIP events of its own
+launch.  (The unit-cost kernels launch one wave per tile: for them that sweep is
+skipped.)  Then it tries running the narrow classes on a wider kernel (fewer
+launches, fewer tails) and running the classes concurrently on 2 or 4 streams,
+timing whole steps.  The result is kept only if it beats the default launch by
+2%.  The winner is cached in saruman_amd/tuning/<arch>.json and applied with
+srm_ctx_set_tuning().
 
     python -m saruman_amd.tune [--reads N]     (on the GPU; default: the 2M-read mixed workload)
 """
@@ -86,8 +88,9 @@ def _total(job, reps):
     return float(np.median([job.run("banded") for _ in range(reps)]))
 
 
-def tune(ctx, upload, reps: int = 3, log=print):
-    """upload(job) uploads the representative candidate batch into a Job."""
+def tune(ctx, upload, reps: int = 3, log=print, unit: bool = True):
+    """upload(job) uploads the representative candidate batch into a Job; unit:
+    the batch runs match 0 / mismatch 1 (no blocks-per-CU sweep)."""
     from .batch import Job
     lib = _lib.load()
     job = Job(ctx)
@@ -97,9 +100,9 @@ def tune(ctx, upload, reps: int = 3, log=print):
         base_total, base = _class_times(job, reps)
         log(f"[tune] default launch: band kernels {base_total:.3f} ms "
             + " ".join(f"nw{nw}={ms:.3f}" for nw, ms in base.items()))
-        occ = {nw: lib.srm_band_occupancy(nw, 1) for nw in range(1, 9)}
+        occ = {nw: lib.srm_band_occupancy(nw, 0) for nw in range(1, 9)}
         best = {nw: (ms, 0) for nw, ms in base.items()}
-        for b in range(1, max(occ.values()) + 1):
+        for b in range(1, 0 if unit else max(occ.values()) + 1):
             t = {"blocks_per_cu": [0] + [min(b, occ[nw]) for nw in range(1, 9)], "merge_below": 0,
                  "class_streams": 1}
             apply(ctx, t)
