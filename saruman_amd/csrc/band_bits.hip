@@ -23,7 +23,10 @@
 //     delta vectors.  A band value is then boundary + deltas + D bits.
 // That is 1 bit per cell plus 3-5 words per block, against 4 bits per cell for
 // the nibble band.  Columns L - 1 and L go to the tile's tail as nibbles, in the
-// format trace_kernel reads (column L has the free trailing rows).
+// format trace_kernel reads (column L has the free trailing rows).  The forward
+// pass also leaves the read's and the band's window codes, 16 per word, lane-
+// interleaved: the walk's lanes then read them a few cache lines per load instead
+// of one line per lane.
 //
 // Walk (bits_trace_kernel): the decisions of cuda-functions.cu:121-276 with unit
 // costs.  A cell steps diagonally iff D0 is 0 (Dg + mismatch == v) or the
@@ -52,6 +55,15 @@ struct BitsShape {
     static constexpr int DWLO = dwords(NLO);
     static constexpr int DW = DWLO + dwords(NHI);        // D words per block
     static constexpr int WB = DW + 1 + 2 * NWD;          // + cell 0's value, +1 deltas, -1 deltas
+    // a tile's scratch in rows of 64 words (word l of a row is lane l's), for a
+    // tile whose longest read is Lmax: blocks (columns 1 .. Lmax-1), tail, read
+    // codes (row u: codes 16u .. 16u+15 of the read), window codes (row r: codes
+    // 16r .. 16r+15 from the code of band cell 0 in column 1)
+    __host__ __device__ static constexpr int blocks(int Lmax) { return Lmax >= 2 ? ((Lmax - 2) >> 3) + 1 : 0; }
+    __host__ __device__ static constexpr int tail_row(int Lmax) { return blocks(Lmax) * WB; }
+    __host__ __device__ static constexpr int read_row(int Lmax) { return tail_row(Lmax) + 2 * NW + 2; }
+    __host__ __device__ static constexpr int win_row(int Lmax) { return read_row(Lmax) + ((Lmax + 31) >> 4) + 1; }
+    __host__ __device__ static constexpr int rows(int Lmax) { return win_row(Lmax) + 2 * (NWD + ((Lmax + 30) >> 5) + 2); }
 };
 
 // word and bit offset of diagonal i's D byte among a block's D words
@@ -183,12 +195,23 @@ void bits_band_kernel(BandArgs a)
         // codes stream through chunks of 32: ch[m] holds codes gbase + 32(m0 + m).
         const int64_t gbase = (int64_t)cd.win_off + dlo;
         const int64_t rbase = (int64_t)cd.read_off;
+        // the window codes also go to the tile's window rows (2 per chunk) for the walk
+        uint32_t *__restrict__ wrow = scr + (size_t)S::win_row(Lmax) * 64 + lane;
+        uint32_t *__restrict__ rrow = scr + (size_t)S::read_row(Lmax) * 64 + lane;
         uint32_t ch0[NWD + 1], ch1[NWD + 1];
 #pragma unroll
         for (int m = 0; m <= NWD; ++m) {
-            ch0[m] = 0u;
-            ch1[m] = 0u;
-            if (active) planes32(a.win_words, gbase + 32 * m, ch0[m], ch1[m]);
+            uint32_t x = 0u, y = 0u;
+            if (active) {
+                const int64_t idx = gbase + 32 * m;
+                const uint32_t *__restrict__ wsrc = a.win_words + (idx >> 4);
+                const uint32_t sh = (uint32_t)(idx & 15) * 2u;
+                x = __builtin_amdgcn_alignbit(wsrc[1], wsrc[0], sh);
+                y = __builtin_amdgcn_alignbit(wsrc[2], wsrc[1], sh);
+            }
+            planes_of(x, y, ch0[m], ch1[m]);
+            __builtin_nontemporal_store(x, &wrow[(2 * m) * 64]);
+            __builtin_nontemporal_store(y, &wrow[(2 * m + 1) * 64]);
         }
         // the read's codes, staged in this wave's LDS for reads of up to 250 codes
         // (lane-interleaved words; word u holds read_words word ((rbase - 1) >> 4) + u):
@@ -241,10 +264,17 @@ void bits_band_kernel(BandArgs a)
                     ch0[m] = ch0[m + 1];
                     ch1[m] = ch1[m + 1];
                 }
-                planes_of(__builtin_amdgcn_alignbit(pg1, pg0, gsh0),
-                          __builtin_amdgcn_alignbit(pg2, pg1, gsh0), ch0[NWD], ch1[NWD]);
+                const uint32_t x = __builtin_amdgcn_alignbit(pg1, pg0, gsh0);
+                const uint32_t y = __builtin_amdgcn_alignbit(pg2, pg1, gsh0);
+                planes_of(x, y, ch0[NWD], ch1[NWD]);
+                const int m = NWD + ((cb - 1) >> 5);
+                __builtin_nontemporal_store(x, &wrow[(2 * m) * 64]);
+                __builtin_nontemporal_store(y, &wrow[(2 * m + 1) * 64]);
             }
-            if (((cb - 1) & 15) == 0) rbuf = read16(cb - 1);
+            if (((cb - 1) & 15) == 0) {
+                rbuf = read16(cb - 1);
+                __builtin_nontemporal_store(rbuf, &rrow[((cb - 1) >> 4) * 64]);
+            }
             store_boundary(q);   // column cb - 1 = 8q (garbage on lanes past their read)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -361,7 +391,7 @@ void bits_band_kernel(BandArgs a)
         // diagonal (nibble j of word m is cell 8m + j, values clipped at K), then
         // the equal-character bits of column L.  Column L - 1 from its deltas;
         // column L with the free top moves of its trailing rows.
-        uint32_t *__restrict__ tail = scr + (size_t)((Lmax + 8) >> 3) * BMAX * 64 + lane;
+        uint32_t *__restrict__ tail = scr + (size_t)S::tail_row(Lmax) * 64 + lane;
         if (active) {
             const uint32_t rc = read16(L - 1) & 3u;
             const uint32_t rrep = rc * 0x55555555u;
@@ -431,7 +461,7 @@ void bits_trace_kernel(BandArgs a)
     // the packed tail (the first run's columns L - 1 and L), loaded with the
     // descriptor rather than after the score test
     const uint32_t *__restrict__ tail =
-        a.scratch + a.tile_scratch[tile] + (size_t)((td.z + 8) >> 3) * BMAX * 64 + lane;
+        a.scratch + a.tile_scratch[tile] + (size_t)S::tail_row((int)td.z) * 64 + lane;
     typedef uint32_t TailWords __attribute__((ext_vector_type(NW)));
     TailWords tl, tp;
 #pragma unroll
@@ -451,7 +481,6 @@ void bits_trace_kernel(BandArgs a)
     const uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
     const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
     uint32_t *__restrict__ ops = a.ops + cd.ops_off;
-    const int64_t rbase = (int64_t)cd.read_off;
     const int cap = w + k;
     int row = w, col = L, n = 0;
     uint32_t opw = 0;
@@ -494,17 +523,10 @@ void bits_trace_kernel(BandArgs a)
         }
         if (!done) v = nib(tp, row - (L - 1) - dlo);   // the cell in column L - 1
     }
-    // the read's codes, staged in this wave's LDS (lane-interleaved words); word u
-    // holds the read_words word ((rbase - 1) >> 4) + u
-    extern __shared__ uint32_t trace_lds[];
-    const int rwn = a.code_words;
-    uint32_t *__restrict__ lcodes = trace_lds + (size_t)(threadIdx.x >> 6) * 64 * rwn + lane;
-    if (rwn && !done) {
-        const uint32_t *__restrict__ src = a.read_words + ((rbase - 1) >> 4);
-        const int nwd = ((L + 15) >> 4) + 3;   // <= rwn: L <= the class's max L
-        for (int u = 0; u < nwd; ++u) lcodes[u * 64] = src[u];
-    }
-    const int64_t woff = (int64_t)cd.win_off;
+    // the read's codes and the band's window codes, as the forward pass left them
+    // in the tile's rows: a segment's words are in the same few rows for the lanes
+    const uint32_t *__restrict__ rrow = scr + (size_t)S::read_row((int)td.z) * 64 + lane;
+    const uint32_t *__restrict__ wrow = scr + (size_t)S::win_row((int)td.z) * 64 + lane;
     // the walk goes by segments of 32 columns (4 blocks) along its diagonal: the D0
     // bits of the diagonal in columns 32g+1 .. 32g+32 and the mismatches of those
     // cells, bit j for column 32g + 1 + j; a diagonal run crosses a segment in one
@@ -565,33 +587,16 @@ void bits_trace_kernel(BandArgs a)
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 dw[u] = scr[((size_t)imin(4 * g + u, qlast) * WB + wd) * 64 + lane];
-            // characters: read codes 32g .. 32g+31 against window codes from
-            // i + dlo + 32g (exact for rows and columns >= 1, the only cells whose
-            // characters can decide)
-            const int64_t wb = woff + (i + dlo + 32 * g);
-            const uint32_t *__restrict__ ww = a.win_words + (wb >> 4);
-            const uint32_t g0 = ww[0], g1 = ww[1], g2 = ww[2];
-            const uint32_t gsh = (uint32_t)(wb & 15) * 2u;
-            uint32_t r0, r1, r2, rsh;
-            if (rwn) {   // code rbase + 32g sits at place o of the staged codes
-                const int o = (int)((rbase - 1) & 15) + 1 + 32 * g;
-                const int wi = o >> 4;
-                r0 = lcodes[wi * 64];
-                r1 = lcodes[(wi + 1) * 64];
-                r2 = lcodes[(wi + 2) * 64];
-                rsh = (uint32_t)(o & 15) * 2u;
-            } else {
-                const int64_t rb = rbase + 32 * g;
-                const uint32_t *__restrict__ rw = a.read_words + (rb >> 4);
-                r0 = rw[0];
-                r1 = rw[1];
-                r2 = rw[2];
-                rsh = (uint32_t)(rb & 15) * 2u;
-            }
-            const uint32_t x0 = __builtin_amdgcn_alignbit(r1, r0, rsh) ^
-                                __builtin_amdgcn_alignbit(g1, g0, gsh);
-            const uint32_t x1 = __builtin_amdgcn_alignbit(r2, r1, rsh) ^
-                                __builtin_amdgcn_alignbit(g2, g1, gsh);
+            // characters: read codes 32g .. 32g+31 (rows 2g, 2g+1) against window
+            // codes from band cell 0's code + i + 32g (exact for rows and columns
+            // >= 1, the only cells whose characters can decide)
+            const uint32_t r0 = rrow[(2 * g) * 64], r1 = rrow[(2 * g + 1) * 64];
+            const int wp = imax(i, 0) + 32 * g;
+            const uint32_t *__restrict__ ww = wrow + (size_t)(wp >> 4) * 64;
+            const uint32_t g0 = ww[0], g1 = ww[64], g2 = ww[128];
+            const uint32_t gsh = (uint32_t)(wp & 15) * 2u;
+            const uint32_t x0 = r0 ^ __builtin_amdgcn_alignbit(g1, g0, gsh);
+            const uint32_t x1 = r1 ^ __builtin_amdgcn_alignbit(g2, g1, gsh);
             mseg = even_bits(x0 | (x0 >> 1)) | (even_bits(x1 | (x1 >> 1)) << 16);
             dseg = 0u;
 #pragma unroll
@@ -655,6 +660,20 @@ void bits_trace_kernel(BandArgs a)
 // ---------------------------------------------------------------------------
 // host-side launch wrappers
 // ---------------------------------------------------------------------------
+// words of one tile's scratch for the unit-cost kernels
+size_t bits_tile_words(int nw, int maxL)
+{
+    switch (nw) {
+    case 1: return (size_t)BitsShape<1>::rows(maxL) * 64;
+    case 2: return (size_t)BitsShape<2>::rows(maxL) * 64;
+    case 3: return (size_t)BitsShape<3>::rows(maxL) * 64;
+    case 4: return (size_t)BitsShape<4>::rows(maxL) * 64;
+    case 5: return (size_t)BitsShape<5>::rows(maxL) * 64;
+    case 6: return (size_t)BitsShape<6>::rows(maxL) * 64;
+    case 7: return (size_t)BitsShape<7>::rows(maxL) * 64;
+    default: return (size_t)BitsShape<8>::rows(maxL) * 64;
+    }
+}
 template <int NW>
 static hipError_t launch_bits_band_nw(const BandArgs &a, int blocks, size_t lds, hipStream_t st)
 {

@@ -83,6 +83,7 @@ hipError_t launch_trace(const BandArgs &a, int nw, bool unit, int maxL, hipStrea
 hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, int maxL, hipStream_t st);
 hipError_t launch_bits_trace(const BandArgs &a, int nw, size_t lds, hipStream_t st);
 int bits_band_occupancy(int nw);
+size_t bits_tile_words(int nw, int maxL);   // one tile's scratch (words)
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);

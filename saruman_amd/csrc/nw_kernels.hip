@@ -652,7 +652,7 @@ hipError_t launch_trace(const BandArgs &args, int nw, bool unit, int maxL, hipSt
     BandArgs a = args;
     a.code_words = maxL <= 600 ? trace_code_words(maxL) : 0;
     const size_t lds = (size_t)a.code_words * 256 * 4;
-    if (unit) return launch_bits_trace(a, nw, lds, st);
+    if (unit) return launch_bits_trace(a, nw, 0, st);   // codes from the tile's rows
     switch (nw) {
     case 1: return launch_trace_nw<1>(a, lds, st);
     case 2: return launch_trace_nw<2>(a, lds, st);

@@ -685,7 +685,10 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             // every tile keeps its band until trace_kernel has walked it
             for (uint32_t u = 0; u < bc.ntiles; ++u) {
                 tile_off.push_back(scratch_words);
-                scratch_words += tile_scratch_words(bc.nw, (int)tiles[4 * (bc.tile_begin + u) + 2]);
+                const int tmax = (int)tiles[4 * (bc.tile_begin + u) + 2];
+                scratch_words += (match == 0 && mismatch == 1)
+                                     ? srm::bits_tile_words(bc.nw, tmax)
+                                     : tile_scratch_words(bc.nw, tmax);
             }
             j->classes.push_back(bc);
             i = e;

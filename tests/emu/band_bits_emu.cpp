@@ -126,7 +126,7 @@ void run_class(std::vector<Case> &cs, Tally &tally)
         maxL = imx(maxL, mL);
         tiles.push_back(uint4{(uint32_t)b, (uint32_t)cnt, (uint32_t)mL, (uint32_t)mk});
         toff.push_back(words);
-        words += ((size_t)NW * 8 * ((mL + 8) / 8) + 2 * (size_t)NW + 2) * 64;
+        words += srm::bits_tile_words(NW, mL);
     }
     std::vector<srm_cand> ctile(n);
     for (int p = 0; p < n; ++p) ctile[p] = cand[perm[p]];
@@ -159,8 +159,8 @@ void run_class(std::vector<Case> &cs, Tally &tally)
             blockDim.x = 256;
             srm::bits_band_kernel<NW>(a);
         }
-    // walk: the LDS staging rule of launch_trace
-    a.code_words = maxL <= 600 ? ((maxL + 15) >> 4) + 3 : 0;
+    // walk (codes from the tile's rows)
+    a.code_words = 0;
     for (uint32_t blk = 0; blk < blocks; ++blk)
         for (uint32_t th = 0; th < 256; ++th) {
             threadIdx.x = th;
