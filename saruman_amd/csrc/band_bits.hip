@@ -1,7 +1,7 @@
 // band_bits.hip -- the banded kernels for unit costs (match 0, mismatch 1, gap 1),
-// the costs of every SARUMAN run (src/saruman.c: match 0, mismatch 1).  Same
-// results as band_kernel / trace_kernel (nw_kernels.hip), from a bit-parallel
-// forward pass.
+// the costs of every SARUMAN run (src/saruman.c: match 0, mismatch 1).  Their
+// results equal the plain kernel's (nw_kernels.hip) bit for bit; other costs run
+// the nibble band of band_kernel / trace_kernel there.
 //
 // Forward pass (bits_band_kernel): with unit costs neighbouring DP cells differ by
 // -1, 0 or +1, so a column of the band is two bit vectors (vertical deltas +1 and
@@ -23,7 +23,7 @@
 //     delta vectors.  A band value is then boundary + deltas + D bits.
 // That is 1 bit per cell plus 3-5 words per block, against 4 bits per cell for
 // the nibble band.  Columns L - 1 and L go to the tile's tail as nibbles, in the
-// format trace_kernel reads (column L has the free trailing rows).  The forward
+// format of trace_kernel's tail (column L has the free trailing rows).  The forward
 // pass also leaves the read's and the band's window codes, 16 per word, lane-
 // interleaved: the walk's lanes then read them a few cache lines per load instead
 // of one line per lane.
@@ -31,9 +31,10 @@
 // Walk (bits_trace_kernel): the decisions of cuda-functions.cu:121-276 with unit
 // costs.  A cell steps diagonally iff D0 is 0 (Dg + mismatch == v) or the
 // characters are equal (Dg == v), so a diagonal run is the run of cells with
-// D0 & mismatch == 0: found for a whole block from one byte.  The walk carries
-// the value v of its cell; the other cells a decision compares (Tp, Lf) are
-// computed from the block's boundary and D bits.
+// D0 & mismatch == 0: found for a whole 32-column segment from one word (the D
+// bytes of its 4 blocks).  The walk carries the value v of its cell; the other
+// cells a decision compares (Tp, Lf) are computed from the block's boundary and
+// D bits.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -726,23 +727,24 @@ int bits_band_occupancy(int nw)
 }
 
 template <int NW>
-static hipError_t launch_bits_trace_nw(const BandArgs &a, size_t lds, hipStream_t st)
+static hipError_t launch_bits_trace_nw(const BandArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(bits_trace_kernel<NW>, dim3((a.ntiles + 3) / 4), dim3(256), lds, st, a);
+    hipLaunchKernelGGL(bits_trace_kernel<NW>, dim3((a.ntiles + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bits_trace(const BandArgs &a, int nw, size_t lds, hipStream_t st)
+// the walk reads the codes from the tile's rows: no LDS
+hipError_t launch_bits_trace(const BandArgs &a, int nw, hipStream_t st)
 {
     switch (nw) {
-    case 1: return launch_bits_trace_nw<1>(a, lds, st);
-    case 2: return launch_bits_trace_nw<2>(a, lds, st);
-    case 3: return launch_bits_trace_nw<3>(a, lds, st);
-    case 4: return launch_bits_trace_nw<4>(a, lds, st);
-    case 5: return launch_bits_trace_nw<5>(a, lds, st);
-    case 6: return launch_bits_trace_nw<6>(a, lds, st);
-    case 7: return launch_bits_trace_nw<7>(a, lds, st);
-    case 8: return launch_bits_trace_nw<8>(a, lds, st);
+    case 1: return launch_bits_trace_nw<1>(a, st);
+    case 2: return launch_bits_trace_nw<2>(a, st);
+    case 3: return launch_bits_trace_nw<3>(a, st);
+    case 4: return launch_bits_trace_nw<4>(a, st);
+    case 5: return launch_bits_trace_nw<5>(a, st);
+    case 6: return launch_bits_trace_nw<6>(a, st);
+    case 7: return launch_bits_trace_nw<7>(a, st);
+    case 8: return launch_bits_trace_nw<8>(a, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -46,8 +46,8 @@ struct BandArgs {
     const uint64_t *tile_scratch;   // [ntiles] word offset of each tile's band in scratch
     srm_result *res;
     uint32_t *ops;
-    int code_words;                 // trace_kernel: LDS words per lane staging the read's
-                                    // codes (0: the codes come from read_words)
+    int code_words;                 // LDS words per lane staging the read's codes
+                                    // (bits_band_kernel, trace_kernel; 0: read_words)
 };
 
 struct RecArgs {
@@ -81,7 +81,7 @@ hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, int max
 hipError_t launch_trace(const BandArgs &a, int nw, bool unit, int maxL, hipStream_t st);
 // unit costs (band_bits.hip): the bit-parallel forward pass and its walk
 hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, int maxL, hipStream_t st);
-hipError_t launch_bits_trace(const BandArgs &a, int nw, size_t lds, hipStream_t st);
+hipError_t launch_bits_trace(const BandArgs &a, int nw, hipStream_t st);
 int bits_band_occupancy(int nw);
 size_t bits_tile_words(int nw, int maxL);   // one tile's scratch (words)
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);

@@ -643,16 +643,16 @@ static hipError_t launch_trace_nw(const BandArgs &a, size_t lds, hipStream_t st)
 // before the read's first to 16 past its last, from word boundaries
 static int trace_code_words(int maxL) { return ((maxL + 15) >> 4) + 3; }
 
-// one wave per tile of the class; unit: match 0, mismatch 1.  Reads of up to
-// 600 codes are staged in LDS (at most 40 KiB per block of 4 waves); longer
-// ones are read from read_words.
+// one wave per tile of the class; unit (match 0, mismatch 1): bits_trace_kernel.
+// trace_kernel stages reads of up to 600 codes in LDS (at most 40 KiB per block
+// of 4 waves); longer ones are read from read_words.
 hipError_t launch_trace(const BandArgs &args, int nw, bool unit, int maxL, hipStream_t st)
 {
     if (args.ntiles == 0) return hipSuccess;
+    if (unit) return launch_bits_trace(args, nw, st);
     BandArgs a = args;
     a.code_words = maxL <= 600 ? trace_code_words(maxL) : 0;
     const size_t lds = (size_t)a.code_words * 256 * 4;
-    if (unit) return launch_bits_trace(a, nw, 0, st);   // codes from the tile's rows
     switch (nw) {
     case 1: return launch_trace_nw<1>(a, lds, st);
     case 2: return launch_trace_nw<2>(a, lds, st);
