@@ -320,6 +320,13 @@ int srm_index_build(const char *genome, size_t len, const int *qs, int nq, int t
                     srm_index **out);
 void srm_index_destroy(srm_index *index);
 
+/* the index of one genome chunk (construct_qgram_index with run > 0,
+ * src/util.c:456-499): offsets 0..len-q of the chunk text, stored as
+ * base + offset (base = chunk number * -G); offset 0 of chunk 0 stays the list
+ * terminator.  The device filter takes only base 0. */
+int srm_index_build_at(const char *chunk_text, size_t len, const int *qs, int nq, int threads,
+                       uint32_t base, srm_index **out);
+
 /*
  * Candidates of one strand (do_reverse 0 or 4) in the reference's batch order
  * (replaces qgram_to_read_mapping/run_seashore/add_start_position/
@@ -331,6 +338,17 @@ int srm_filter_run(const srm_index *index, const char *genome, size_t glen,
                    const uint8_t *blocked, const char *reads, const uint64_t *read_off,
                    const uint16_t *read_len, const uint8_t *read_k, size_t n_reads,
                    int do_reverse, int chunk_value, int threads, srm_filter **out);
+/*
+ * srm_filter_run over one genome chunk (the reference's pass genome_chunk_run of
+ * -G mode, src/hash.c:545-552, 982, 998): positions are genome positions
+ * (chunk offset plus the index's base), a read start must lie after the chunk's
+ * first base, the perfect check reads chunk_text (its overhang included), and a
+ * window is queued only if it ends at or before `limit` (total_bases_read).
+ */
+int srm_filter_run_chunk(const srm_index *index, const char *chunk_text, size_t text_len,
+                         uint64_t limit, const char *reads, const uint64_t *read_off,
+                         const uint16_t *read_len, const uint8_t *read_k, size_t n_reads,
+                         int do_reverse, int chunk_value, int threads, srm_filter **out);
 int srm_filter_sizes(const srm_filter *f, uint64_t *n_cand, uint64_t *n_batches,
                      uint64_t *n_perfect);
 int srm_filter_copy(const srm_filter *f, int32_t *cand_read, int64_t *cand_pos,
@@ -381,6 +399,21 @@ int srm_fastx_sizes(const srm_fastx *x, uint64_t *n_records, uint64_t *n_bases,
                     uint64_t *skipped_short, uint64_t *skipped_n);
 int srm_fastx_copy(const srm_fastx *x, char *seq, uint64_t *off, uint32_t *len, char *names);
 void srm_fastx_destroy(srm_fastx *x);
+
+/*
+ * Genome chunks of the reference's -G mode (replaces the genome reading loop of
+ * src/saruman.c:139-271 and its end test at :415): chunk i holds up to `chunk`
+ * bases plus an overhang of up to read_len bases, exactly as the reference's
+ * reader forms them (its header skip, buffer re-use and end-of-file byte
+ * included).  Its q-gram positions start at i * chunk; limit[i] is the
+ * reference's total_bases_read after chunk i, which bounds its windows.
+ */
+typedef struct srm_gchunks srm_gchunks;
+int srm_genome_chunks_load(const char *path, uint32_t chunk, int read_len, srm_gchunks **out);
+int srm_genome_chunks_sizes(const srm_gchunks *x, uint64_t *n_chunks, uint64_t *n_bytes);
+/* text: n_bytes; off: n_chunks + 1 (chunk i is text[off[i], off[i+1])); limit: n_chunks */
+int srm_genome_chunks_copy(const srm_gchunks *x, char *text, uint64_t *off, uint64_t *limit);
+void srm_genome_chunks_destroy(srm_gchunks *x);
 
 /* ===================================================================== */
 /* Part 4. Serving: continuous batching over a resident genome            */

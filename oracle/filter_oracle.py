@@ -14,6 +14,16 @@ Restated reference code (read as text):
   - cuda_queue_check              src/hash.c:841-851
   - prepare_cuda_alignments       src/hash.c:898-1111 (batch layout)
 
+-G chunks (src/saruman.c:139-415): build_index(base=) and run_strand(base=,
+limit=) restate one pass of the chunk loop: q-gram positions are
+base + offset (util.c:481), a read start must lie after the chunk's first base
+(hash.c:545), the perfect check reads the chunk text (hash.c:552), a window must
+end at or before total_bases_read (hash.c:982) and is read from the chunk at
+position - base (hash.c:998).  A start-free window of chunk > 0 may begin up to
+k bases before the chunk: the reference then indexes its chunk buffer with a
+negative offset (in unsigned arithmetic), which is undefined; this restatement,
+and the product, take those bases from the genome (window_genome).
+
 Quirks kept on purpose: Q10 (s/e leak after an out-of-genome skip), Q11 (flag
 branches exactly as written), Q12 (the first-pass scan of list b starts at the
 index of P in list a, then 1, 2, ...), Q13 (only the exact_tresh-th matching
@@ -60,11 +70,12 @@ def derive_params(L: int, k: int) -> Params:
     return Params(L, k, q, rest, n, pos, exact_tresh)
 
 
-def build_index(genome: bytes, q: int) -> dict:
-    """util.c:474-481: every offset 0..len-q, ascending, keyed by the q bytes."""
+def build_index(genome: bytes, q: int, base: int = 0) -> dict:
+    """util.c:474-481: every offset 0..len-q, ascending, keyed by the q bytes,
+    stored as base + offset (base = chunk number * G)."""
     idx: dict = {}
     for p in range(0, len(genome) - q + 1):
-        idx.setdefault(genome[p:p + q], []).append(p)
+        idx.setdefault(genome[p:p + q], []).append(p + base)
     return idx
 
 
@@ -135,7 +146,7 @@ def _offsets(flag, k):
 
 
 def prepare_batch(strand: Strand, genome: bytes, reads: list, L: int, k: int, do_reverse: int,
-                  total_bases: int):
+                  total_bases: int, base: int = 0, window_genome: bytes | None = None):
     """hash.c:898-1111 -- returns the list of candidates of one call (may be empty)."""
     out = []
     if strand.queue <= 0:
@@ -158,7 +169,7 @@ def prepare_batch(strand: Strand, genome: bytes, reads: list, L: int, k: int, do
                 if L + start_offset + end_offset != L + (s or 0) + (e or 0):
                     STATS["leak"] += 1
                 w = L + start_offset + end_offset
-                win = genome[p:p + w]
+                win = genome[p - base:p - base + w] if p >= base else window_genome[p:p + w]
                 win = win.split(b"\x00")[0]
                 win = win + b"\x00" * (w - len(win))   # strncpy zero pad
                 rd = reads[rid]
@@ -174,7 +185,8 @@ def prepare_batch(strand: Strand, genome: bytes, reads: list, L: int, k: int, do
 
 
 def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_reverse: int,
-               chunk_value: int, index: dict | None = None):
+               chunk_value: int, index: dict | None = None, base: int = 0,
+               limit: int | None = None, window_genome: bytes | None = None):
     """qgram_to_read_mapping + run_seashore for one strand.
 
     Returns (batches, perfect_records) where batches is a list of candidate lists
@@ -185,7 +197,8 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
     q, n, rest = P.q, P.n, P.rest
     half = n // 2
     if index is None:
-        index = build_index(genome, q)
+        index = build_index(genome, q, base)
+    total_bases = len(genome) if limit is None else limit
     strand = Strand()
     batches = []
     perfect = []
@@ -241,8 +254,8 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
                 Pa = D(offs[counter + a] + off)
                 while Pa > 0:
                     gs = Pa - in_read
-                    if gs + 1 > 0:
-                        tg = genome[gs + 1:gs + 1 + L]
+                    if gs + 1 - base > 0:
+                        tg = genome[gs + 1 - base:gs + 1 - base + L]
                         tg = tg.split(b"\x00")[0]
                         if tg == tmp_read:
                             if strand.add_start_position(read_id, -(gs + 1), 0, P.exact_tresh,
@@ -307,14 +320,16 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
                     Pa = D(offs[counter + a] + off)
             # cuda_queue_check, hash.c:835, 841-851
             if strand.queue > chunk_value:
-                b = prepare_batch(strand, genome, reads, L, k, do_reverse, len(genome))
+                b = prepare_batch(strand, genome, reads, L, k, do_reverse, total_bases, base,
+                                  window_genome)
                 if b:
                     batches.append(b)
                 strand.queue = 0
             read_id += 1
             counter2 += 1
             counter += n
-    b = prepare_batch(strand, genome, reads, L, k, do_reverse, len(genome))   # hash.c:468
+    b = prepare_batch(strand, genome, reads, L, k, do_reverse, total_bases, base,
+                      window_genome)   # hash.c:468
     if b:
         batches.append(b)
     return batches, perfect

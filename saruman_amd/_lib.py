@@ -67,7 +67,11 @@ _PROTOS = [
     # host candidate generation and synthetic workloads
     ("srm_filter_params", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     ("srm_index_build", C.c_int, [_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.POINTER(_vp)]),
+    ("srm_index_build_at", C.c_int, [_vp, C.c_size_t, _vp, C.c_int, C.c_int, C.c_uint32,
+                                     C.POINTER(_vp)]),
     ("srm_index_destroy", None, [_vp]),
+    ("srm_filter_run_chunk", C.c_int, [_vp, _vp, C.c_size_t, C.c_uint64, _vp, _vp, _vp, _vp,
+                                       C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("srm_filter_run", C.c_int, [_vp, _vp, C.c_size_t, _vp, _vp, _vp, _vp, _vp, C.c_size_t,
                                  C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("srm_filter_sizes", C.c_int, [_vp, _vp, _vp, _vp]),
@@ -93,6 +97,10 @@ _PROTOS = [
     ("srm_fastx_sizes", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("srm_fastx_copy", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("srm_fastx_destroy", None, [_vp]),
+    ("srm_genome_chunks_load", C.c_int, [C.c_char_p, C.c_uint32, C.c_int, C.POINTER(_vp)]),
+    ("srm_genome_chunks_sizes", C.c_int, [_vp, _vp, _vp]),
+    ("srm_genome_chunks_copy", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("srm_genome_chunks_destroy", None, [_vp]),
     # serving
     ("srm_server_create", C.c_int, [_vp, _vp, C.c_size_t, _vp, C.c_int, C.POINTER(_vp)]),
     ("srm_server_submit", C.c_longlong, [_vp, _vp, _vp, _vp, _vp, C.c_size_t]),

@@ -174,8 +174,11 @@ struct Lists {
         }
         const QIndex *X = ix->get(P.q);
         for (int x = 0; x < P.n; ++x) {
-            c.start.push_back((uint32_t)c.flat.size());
+            const size_t from = c.flat.size();
+            c.start.push_back((uint32_t)from);
             if (X) X->lookup(oriented.data() + P.gram[(size_t)x], c.flat);
+            if (ix->base)   // a chunk's positions: base + offset (util.c:481)
+                for (size_t u = from; u < c.flat.size(); ++u) c.flat[u] += ix->base;
             c.flat.push_back(0);
         }
         return c;
@@ -221,9 +224,10 @@ int add_start(ReadOut &ro, int32_t pos, int flag, int possible, int exact_tresh,
     return 2;
 }
 
-// run_seashore for one read (hash.c:502-834)
+// run_seashore for one read (hash.c:502-834).  genome/glen: the text the perfect
+// check reads, whose offset 0 is genome position `base` (a -G chunk, hash.c:545-552)
 void seashore_read(Lists &ls, size_t rid, const Params &P, const char *genome, size_t glen,
-                   int do_reverse, ReadOut &ro)
+                   long base, int do_reverse, ReadOut &ro)
 {
     const int q = P.q, half = P.half, k = P.k, L = P.L, rest = P.rest, dr = do_reverse;
     const char *rd = ls.reads + ls.roff[rid];
@@ -239,11 +243,11 @@ void seashore_read(Lists &ls, size_t rid, const Params &P, const char *genome, s
         uint32_t Pa = ls.D(rid, a, off);
         while (Pa > 0) {
             const long gs = (long)Pa - in_read;
-            if (gs + 1 > 0) {
+            if (gs + 1 - base > 0) {
                 // perfect check (strncpy + strcmp, hash.c:550-583)
-                const long S = gs + 1;
-                bool perfect = (size_t)S + (size_t)L <= glen &&
-                               std::memcmp(genome + S, tmp_read.data(), (size_t)L) == 0;
+                const long S = gs + 1, Sl = S - base;
+                bool perfect = (size_t)Sl + (size_t)L <= glen &&
+                               std::memcmp(genome + Sl, tmp_read.data(), (size_t)L) == 0;
                 if (perfect) {
                     // strcmp stops at a NUL in the genome copy: the genome has none
                     if (add_start(ro, (int32_t)(-S), 0, P.exact_tresh, P.exact_tresh, k) != 1)
@@ -349,14 +353,21 @@ int srm_filter_params(int L, int k, int *q, int *rest, int *n, int *exact_tresh)
 int srm_index_build(const char *genome, size_t len, const int *qs, int nq, int threads,
                     srm_index **out)
 {
+    return srm_index_build_at(genome, len, qs, nq, threads, 0, out);
+}
+
+int srm_index_build_at(const char *genome, size_t len, const int *qs, int nq, int threads,
+                       uint32_t base, srm_index **out)
+{
     *out = nullptr;
     int pb = 1;
     while (((size_t)1 << pb) <= len) ++pb;
     for (int i = 0; i < nq; ++i)
         if (qs[i] < 1 || qs[i] > 31 || 2 * qs[i] + pb > 64) return SRM_EINVAL;
-    if (len >= ((size_t)1 << 31)) return SRM_EINVAL;
+    if (len >= ((size_t)1 << 31) || (uint64_t)base + len >= ((uint64_t)1 << 31)) return SRM_EINVAL;
     srm_index *X = new srm_index();
     X->glen = len;
+    X->base = base;
     const int T = nthreads(threads);
     std::vector<int> uq(qs, qs + nq);
     std::sort(uq.begin(), uq.end());
@@ -377,10 +388,12 @@ void srm_index_destroy(srm_index *X) { delete X; }
  * flushes a batch.  blocked (may be NULL): per-genome-base mask of separator
  * bases; a window covering one is skipped like an out-of-genome window.
  */
-int srm_filter_run(const srm_index *ix, const char *genome, size_t glen, const uint8_t *blocked,
-                   const char *reads, const uint64_t *read_off, const uint16_t *read_len,
-                   const uint8_t *read_k, size_t n_reads, int do_reverse, int chunk_value,
-                   int threads, srm_filter **out)
+namespace {
+
+int filter_run(const srm_index *ix, const char *genome, size_t glen, uint64_t limit,
+               const uint8_t *blocked, const char *reads, const uint64_t *read_off,
+               const uint16_t *read_len, const uint8_t *read_k, size_t n_reads, int do_reverse,
+               int chunk_value, int threads, srm_filter **out)
 {
     *out = nullptr;
     if (chunk_value < 1 || (do_reverse != 0 && do_reverse != 4)) return SRM_EINVAL;
@@ -423,7 +436,8 @@ int srm_filter_run(const srm_index *ix, const char *genome, size_t glen, const u
                     size_t run = r / (size_t)chunk_value;
                     ls.chunk_lo = run * (size_t)chunk_value;
                     ls.chunk_hi = std::min(n_reads, (run + 1) * (size_t)chunk_value);
-                    seashore_read(ls, r, params[(size_t)pidx[r]], genome, glen, do_reverse, ro[r]);
+                    seashore_read(ls, r, params[(size_t)pidx[r]], genome, glen, (long)ix->base,
+                                  do_reverse, ro[r]);
                 }
             }
         });
@@ -439,9 +453,39 @@ int srm_filter_run(const srm_index *ix, const char *genome, size_t glen, const u
         int32_t perf_at(size_t r, size_t i) const { return ro[r].perfect[i]; }
         int queued(size_t r) const { return ro[r].queued; }
     };
-    srm_filter *F = batch_pass(Acc{ro}, n_reads, read_len, read_k, glen, blocked, chunk_value);
+    srm_filter *F = batch_pass(Acc{ro}, n_reads, read_len, read_k, limit, blocked, chunk_value);
     *out = F;
     return SRM_OK;
+}
+
+}  // namespace
+
+int srm_filter_run(const srm_index *ix, const char *genome, size_t glen, const uint8_t *blocked,
+                   const char *reads, const uint64_t *read_off, const uint16_t *read_len,
+                   const uint8_t *read_k, size_t n_reads, int do_reverse, int chunk_value,
+                   int threads, srm_filter **out)
+{
+    *out = nullptr;
+    if (ix->base != 0) return SRM_EINVAL;   // a chunk's index: srm_filter_run_chunk
+    return filter_run(ix, genome, glen, glen, blocked, reads, read_off, read_len, read_k, n_reads,
+                      do_reverse, chunk_value, threads, out);
+}
+
+/*
+ * One -G chunk (hash.c:545-552, 982, 998): the index holds genome positions
+ * (its base plus the chunk offset); the perfect check reads chunk_text at
+ * position - base; a window is queued only if it ends at or before limit, the
+ * reference's total_bases_read (so never in the overhang of a full chunk).
+ */
+int srm_filter_run_chunk(const srm_index *ix, const char *chunk_text, size_t text_len,
+                         uint64_t limit, const char *reads, const uint64_t *read_off,
+                         const uint16_t *read_len, const uint8_t *read_k, size_t n_reads,
+                         int do_reverse, int chunk_value, int threads, srm_filter **out)
+{
+    *out = nullptr;
+    if (limit < ix->base) return SRM_EINVAL;
+    return filter_run(ix, chunk_text, text_len, limit, nullptr, reads, read_off, read_len, read_k,
+                      n_reads, do_reverse, chunk_value, threads, out);
 }
 
 int srm_filter_sizes(const srm_filter *F, uint64_t *n_cand, uint64_t *n_batches,

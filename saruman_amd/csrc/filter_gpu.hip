@@ -398,6 +398,8 @@ int srm_dfilter_create(srm_ctx *ctx, const srm_index *ix, const char *genome, si
     *out = nullptr;
     if (!ctx || !ix || glen != ix->glen)
         return srm::set_error(SRM_EINVAL, "genome length does not match the index");
+    if (ix->base != 0)
+        return srm::set_error(SRM_EINVAL, "a -G chunk's index runs on the host filter");
     auto *G = new srm_dfilter();
     G->ctx = ctx;
     G->device = srm::ctx_device(ctx);

@@ -106,6 +106,7 @@ inline Params derive(int L, int k)
 struct srm_index {
     std::vector<srm_filter_impl::QIndex> idx;
     size_t glen = 0;
+    uint32_t base = 0;   // genome position of offset 0 (a -G chunk's start)
     const srm_filter_impl::QIndex *get(int q) const
     {
         for (auto &x : idx)

@@ -8,6 +8,7 @@ of at most 59 characters; with fixed_len, shorter reads skipped, longer cut).
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -65,3 +66,41 @@ def load_reads(path, fixed_len: int = 0, error_rate: float = 0.05, k_rule: str =
     r = Reads(seq, off.astype(np.uint64), ln.astype(np.uint16), np.full(len(off), -1, np.int64),
               np.zeros(len(off), np.uint8), k)
     return r, names, skipped
+
+
+@dataclass
+class GenomeChunk:
+    """One pass of the reference's -G reader: `text` (its G part, then the
+    overhang), `base` (genome position of text[0]) and `limit` (the reference's
+    total_bases_read after this pass: windows end at or before it)."""
+    text: bytes
+    base: int
+    limit: int
+
+
+def load_genome_chunks(path, chunk: int, read_len: int) -> list:
+    """The genome file cut into chunks of `chunk` bases as the reference's -G
+    reader does it, overhang of read_len bases included (srm_genome_chunks_load;
+    src/saruman.c:139-271, 415)."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    check(lib.srm_genome_chunks_load(str(path).encode(), int(chunk), int(read_len), C.byref(h)),
+          f"srm_genome_chunks_load({path})")
+    try:
+        n, b = C.c_uint64(), C.c_uint64()
+        lib.srm_genome_chunks_sizes(h, C.byref(n), C.byref(b))
+        text = np.zeros(max(b.value, 1), np.uint8)
+        off = np.zeros(n.value + 1, np.uint64)
+        lim = np.zeros(max(n.value, 1), np.uint64)
+        lib.srm_genome_chunks_copy(h, ptr(text), ptr(off), ptr(lim))
+    finally:
+        lib.srm_genome_chunks_destroy(h)
+    raw = text[:b.value].tobytes()
+    return [GenomeChunk(raw[int(off[i]):int(off[i + 1])], i * int(chunk), int(lim[i]))
+            for i in range(n.value)]
+
+
+def chunk_genome(chunks) -> bytes:
+    """Genome positions 0..limit of the last chunk: each chunk's bases up to its
+    limit (no window of any chunk reads past that)."""
+    return b"".join(c.text[:c.limit - c.base] for c in chunks)

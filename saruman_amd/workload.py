@@ -136,6 +136,22 @@ class Index:
             pass
 
 
+class ChunkIndex(Index):
+    """The q-gram index of one -G chunk: positions base + offset over the chunk
+    text (srm_index_build_at)."""
+
+    def __init__(self, text: bytes, qs, base: int, threads=0):
+        self.lib = lib()
+        self.qs = sorted(set(int(q) for q in qs))
+        self.base = int(base)
+        h = _vp()
+        qa = np.array(self.qs, np.int32)
+        self._g = np.frombuffer(text, np.uint8)
+        check(self.lib.srm_index_build_at(ptr(self._g), len(self._g), ptr(qa), len(qa), threads,
+                                          self.base, C.byref(h)), "srm_index_build_at")
+        self.h = h
+
+
 @dataclass
 class StrandCands:
     do_reverse: int
@@ -224,6 +240,22 @@ def run_filter(index: Index, genome: Genome, reads: Reads, do_reverse: int, chun
     check(L.srm_filter_run(index.h, ptr(g), len(g), ptr(genome.blocked), ptr(rs), ptr(off),
                            ptr(ln), ptr(kk), reads.n, do_reverse, chunk_value, threads,
                            C.byref(h)), "srm_filter_run")
+    return _collect(L, h, do_reverse, time.perf_counter() - t0)
+
+
+def run_filter_chunk(index: ChunkIndex, text: bytes, limit: int, reads: Reads, do_reverse: int,
+                     chunk_value: int, threads: int = 0) -> StrandCands:
+    """Host filter of one strand over one -G chunk (srm_filter_run_chunk):
+    positions are genome positions; windows end at or before `limit`."""
+    import time
+    L = index.lib
+    h = _vp()
+    g = np.frombuffer(text, np.uint8)
+    rs, off, ln, kk = _reads_args(reads)
+    t0 = time.perf_counter()
+    check(L.srm_filter_run_chunk(index.h, ptr(g), len(g), int(limit), ptr(rs), ptr(off), ptr(ln),
+                                 ptr(kk), reads.n, do_reverse, chunk_value, threads, C.byref(h)),
+          "srm_filter_run_chunk")
     return _collect(L, h, do_reverse, time.perf_counter() - t0)
 
 

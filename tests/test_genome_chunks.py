@@ -1,0 +1,195 @@
+"""-G genome chunks (the reference's chunked genome reader, src/saruman.c:139-271,
+415; the q-gram filter over one chunk, src/hash.c:545-552, 982, 998;
+src/util.c:481).
+
+- The reader (srm_genome_chunks_load) against hand-worked files, one per quirk:
+  the header's '>' appended to a genome that ends inside the first bulk read, the
+  last base appended twice when a single-byte read meets the end of the file, the
+  overhang that skips the bytes a later pass re-uses.  These expected chunks were
+  worked out by hand from the reference's loop.
+- The reader against the closed form of the common case (multi-line FASTA with
+  G well above the line width times L: chunk c is bases [cG, cG + G + L)), and
+  against the Python restatement (oracle/genome_chunks.py) on random files.
+- The chunk filter (srm_filter_run_chunk) against the Python restatement of the
+  filter run over the same chunk (oracle/filter_oracle.py with base/limit); with
+  one chunk covering the genome it is the plain filter.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import filter_oracle as fo
+from oracle import genome_chunks as ogc
+from saruman_amd import ingest
+from saruman_amd import workload as wl
+
+
+def chunks_of(tmp_path, data: bytes, G: int, L: int):
+    p = tmp_path / "g.fa"
+    p.write_bytes(data)
+    return [(c.text, c.base, c.limit) for c in ingest.load_genome_chunks(p, G, L)]
+
+
+def test_genome_inside_first_read_gains_header_byte(tmp_path):
+    want = [(b"ACGTACGTAC>", 0, 11)]
+    data = b">h\nacgtACGTac\n"
+    assert chunks_of(tmp_path, data, 100, 4) == want
+    assert ogc.read_chunks(data, 100, 4) == want
+
+
+def test_last_base_twice_at_end_of_file(tmp_path):
+    data = b">h\nA\nC\nG\nT\nA\nC\nG\nT"
+    want = [(b"ACGTACGTT", 0, 9)]
+    assert chunks_of(tmp_path, data, 10, 1) == want
+    assert ogc.read_chunks(data, 10, 1) == want
+
+
+def test_single_line_overhang_skips_reused_bytes(tmp_path):
+    data = b">h\nabcdefghijklmnopqrstuvwxyz0123\n"
+    want = [(b"ABCDEFGHIJLMNO", 0, 10), (b"KLMNOPQRSTZ012", 10, 20),
+            (b"UVWXYZ0123Z", 20, 30), (b"Z", 30, 31)]
+    assert chunks_of(tmp_path, data, 10, 4) == want
+    assert ogc.read_chunks(data, 10, 4) == want
+
+
+def fasta(bases: bytes, width: int, header=b">chr test", trailing=True) -> bytes:
+    lines = [bases[i:i + width] for i in range(0, len(bases), width)]
+    return header + b"\n" + b"\n".join(lines) + (b"\n" if trailing else b"")
+
+
+def test_multiline_closed_form(tmp_path):
+    rng = np.random.default_rng(5)
+    bases = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 3500))
+    G, L = 1000, 10
+    got = chunks_of(tmp_path, fasta(bases, 60), G, L)
+    assert len(got) == 4
+    for c, (text, base, limit) in enumerate(got[:3]):
+        assert (text, base, limit) == (bases[c * G:c * G + G + L], c * G, (c + 1) * G)
+    text, base, limit = got[3]
+    assert base == 3000 and text == bases[3000:] and limit == 3500   # ends on '\n'
+
+
+def test_reader_matches_restatement_on_random_files(tmp_path):
+    rnd = random.Random(11)
+    n_cases = 0
+    for case in range(300):
+        n = rnd.randrange(0, 400)
+        alphabet = b"ACGTacgtN" if case % 3 else b"ACGT"
+        bases = bytes(rnd.choice(alphabet) for _ in range(n))
+        width = rnd.choice([1, 3, 7, 60, 80, 1000])
+        data = fasta(bases, width, header=rnd.choice([b">h", b">chr1 some text", b""]),
+                     trailing=rnd.random() < 0.7)
+        if case % 17 == 0:
+            data += b">second\nACGT\n"     # a later header is read as sequence
+        G, L = rnd.randrange(2, 120), rnd.randrange(0, 30)
+        assert chunks_of(tmp_path, data, G, L) == ogc.read_chunks(data, G, L), (case, G, L)
+        n_cases += 1
+    assert n_cases == 300
+
+
+def chunk_reads(seed, genome: bytes, n, L, k, at=()):
+    """Synthetic reads, then two reads copied from each start in `at`, one with a
+    substitution in its middle and one with a substitution in its first q-gram
+    (which makes its candidates start-free, flag 3/7)."""
+    g = wl.Genome(genome, [(0, len(genome))], None)
+    reads = wl.synth_reads(seed, g, n, L, L, 0, max_err=k, p_sub=0.8, p_ins=0.1, k=k)
+    if not at:
+        return reads
+    extra = []
+    for p in at:
+        for j in (L // 2, 2):
+            t = bytearray(genome[p:p + L])
+            t[j] = ord("A") if t[j] != ord("A") else ord("C")
+            extra.append(bytes(t))
+    m = len(extra)
+    return wl.Reads(reads.seq + b"".join(extra),
+                    np.concatenate([reads.off, len(reads.seq) + L * np.arange(m)]).astype(np.uint64),
+                    np.full(n + m, L, np.uint16), np.full(n + m, -1, np.int64),
+                    np.zeros(n + m, np.uint8), np.full(n + m, k, np.uint8))
+
+
+CHUNK_STATS = {"before_base": 0}
+
+
+def compare_chunk(text, base, limit, whole, reads, L, k, chunk_value):
+    q = wl.filter_params(L, k)[0]
+    rl = [reads.seq[int(o):int(o) + L] for o in reads.off]
+    names = [f"r{i}" for i in range(reads.n)]
+    idx = wl.ChunkIndex(text, [q], base)
+    oidx = fo.build_index(text, q, base)
+    total = 0
+    try:
+        for dr in (0, 4):
+            want_b, want_p = fo.run_strand(text, rl, names, L, k, dr, chunk_value, oidx, base=base,
+                                           limit=limit, window_genome=whole)
+            got = wl.run_filter_chunk(idx, text, limit, reads, dr, chunk_value)
+            CHUNK_STATS["before_base"] += int((got.pos < base).sum())
+            ends = [0] + got.batch_end.tolist()
+            assert len(ends) - 1 == len(want_b)
+            for bi, wb in enumerate(want_b):
+                lo, hi = ends[bi], ends[bi + 1]
+                g = list(zip(got.read[lo:hi].tolist(), got.pos[lo:hi].tolist(),
+                             got.flag[lo:hi].tolist(), got.w[lo:hi].tolist()))
+                assert g == [(c.read_id, c.pos, c.flag, c.w) for c in wb], (dr, bi)
+                total += len(wb)
+            gp = sorted(zip(got.perfect_read.tolist(), got.perfect_pos.tolist()))
+            wp = sorted((int(s.split("\t")[0][1:]), int(s.split("\t")[1])) for s in want_p)
+            assert gp == wp
+    finally:
+        idx.close()
+    return total
+
+
+def test_chunk_filter_matches_restatement(tmp_path):
+    rng = np.random.default_rng(9)
+    bases = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 24_000))
+    G, L, k = 7_000, 40, 3
+    chunks = chunks_of(tmp_path, fasta(bases, 70), G, L)
+    assert len(chunks) == 4
+    whole = ingest.chunk_genome([ingest.GenomeChunk(*c) for c in chunks])
+    assert whole == bases
+    # reads that start just before, at and after each chunk start, and reads whose
+    # windows end just before or after each chunk's limit
+    at = [B + d for B in (G, 2 * G, 3 * G) for d in range(-L - 3, 6)]
+    at += [B - L - j for B in (G, 2 * G, 3 * G) for j in range(-2, 8)]
+    reads = chunk_reads(17, bases, 600, L, k, at)
+    skips = fo.STATS["skip"]
+    CHUNK_STATS["before_base"] = 0
+    total = 0
+    for text, base, limit in chunks:
+        total += compare_chunk(text, base, limit, whole, reads, L, k, chunk_value=80)
+    assert total > 300
+    assert fo.STATS["skip"] > skips      # windows past a chunk's limit were not queued
+    assert CHUNK_STATS["before_base"] > 0   # start-free windows that begin before their chunk
+
+
+def test_one_chunk_is_the_plain_filter(tmp_path):
+    rng = np.random.default_rng(3)
+    bases = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), 12_000))
+    (text, base, limit), = chunks_of(tmp_path, fasta(bases, 60), 50_000, 36)
+    assert text == bases + b">" and limit == len(bases) + 1
+    reads = chunk_reads(23, bases, 300, 36, 2)
+    q = wl.filter_params(36, 2)[0]
+    g = wl.Genome(text, [(0, len(text))], None)
+    plain_idx = wl.Index(g, [q])
+    idx = wl.ChunkIndex(text, [q], 0)
+    try:
+        for dr in (0, 4):
+            a = wl.run_filter(plain_idx, g, reads, dr, 50)
+            b = wl.run_filter_chunk(idx, text, limit, reads, dr, 50)
+            for f in ("read", "pos", "flag", "w", "batch_end", "perfect_read", "perfect_pos"):
+                assert (getattr(a, f) == getattr(b, f)).all(), f
+    finally:
+        plain_idx.close()
+        idx.close()
+
+
+def test_chunk_index_rejected_by_plain_filter():
+    idx = wl.ChunkIndex(b"ACGT" * 100, [11], 1000)
+    reads = chunk_reads(1, b"ACGT" * 100, 4, 36, 2)
+    try:
+        with pytest.raises(RuntimeError):
+            wl.run_filter(idx, wl.Genome(b"ACGT" * 100, [(0, 400)], None), reads, 0, 10)
+    finally:
+        idx.close()
