@@ -566,6 +566,28 @@ void bits_trace_kernel(BandArgs a)
     };
     while (!done) {
         if (stop_row0 && row == 0) break;
+        if (col == 0 && L >= 2) {
+            // column 0 (Q7, Q4, Q5) in one step: decide() takes the top move while
+            // Tp + 1 - corr == M[row][0] = max(0, row - 2s), with corr = 1 for rows
+            // <= 2s + 3 when s > 0.  So it climbs to row 0 from rows <= 2s (every row
+            // when s = 0), stops at once on rows 2s+1 .. 2s+3, climbs to row 2s + 3
+            // from lower rows; then it stops (row 0 has no top cell)
+            int T = (s == 0 || row <= 2 * s) ? row : imax(row - (2 * s + 3), 0);
+            T = imin(T, cap - n);
+            while (T > 0) {
+                const int c = imin(T, 16 - (n & 15));
+                const uint32_t tops = c == 16 ? 0x55555555u : (1u << (2 * c)) / 3u;
+                opw |= tops << ((n & 15) * 2);
+                n += c;
+                row -= c;
+                T -= c;
+                if ((n & 15) == 0) {
+                    ops[(n >> 4) - 1] = opw;
+                    opw = 0;
+                }
+            }
+            break;
+        }
         if (col == 0) {
             // column 0: M[h][0] = max(0, h - 2s) (Q7); a vertical step or the stop
             // remain (Dg = Lf = COL0)
