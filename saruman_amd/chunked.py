@@ -55,13 +55,17 @@ def map_genome_chunks(chunks: list, reads: wl.Reads, chunk_value: int, names=Non
     seq = reads.seq if isinstance(reads.seq, bytes) else bytes(reads.seq)
     fwd = [seq[int(o):int(o) + int(n)] for o, n in zip(reads.off, reads.length)]
     rev = [_revcomp(r) for r in fwd]
-    out = []
+    # construct_qgram_index ends the reference's run at a pass of no more than q
+    # bases (util.c:466-469), after the records of the passes before it; here the
+    # call fails before mapping anything
     for ch in chunks:
         n = _strlen(ch.text)
         if n <= max(qs):
-            # construct_qgram_index ends the reference here (util.c:466-469)
-            raise ValueError(f"genome chunk at {ch.base} has {n} bases, not more than q = "
+            raise ValueError(f"genome pass at {ch.base} has {n} bases, not more than q = "
                              f"{max(qs)}")
+    out = []
+    for ch in chunks:
+        n = _strlen(ch.text)
         text = ch.text[:n]
         index = wl.ChunkIndex(text, qs, ch.base, threads)
         try:

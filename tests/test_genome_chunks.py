@@ -193,3 +193,25 @@ def test_chunk_index_rejected_by_plain_filter():
             wl.run_filter(idx, wl.Genome(b"ACGT" * 100, [(0, 400)], None), reads, 0, 10)
     finally:
         idx.close()
+
+
+def test_reader_rejects_bad_arguments(tmp_path):
+    p = tmp_path / "g.fa"
+    p.write_bytes(b">h\nACGT\n")
+    with pytest.raises(RuntimeError):
+        ingest.load_genome_chunks(p, 0, 10)            # G = 0 never ends in the reference
+    with pytest.raises(RuntimeError):
+        ingest.load_genome_chunks(tmp_path / "missing.fa", 100, 10)
+
+
+def test_pass_shorter_than_q_fails_before_mapping(tmp_path):
+    from saruman_amd import chunked
+    # the last pass of this file is one base (see the single-line test above)
+    data = b">h\n" + b"ACGTTGCAAC" * 3 + b"\n"
+    p = tmp_path / "g.fa"
+    p.write_bytes(data)
+    chunks = ingest.load_genome_chunks(p, 10, 4)
+    assert len(chunks[-1].text) == 1
+    reads = chunk_reads(5, b"ACGTTGCAAC" * 30, 4, 36, 2)
+    with pytest.raises(ValueError, match="not more than q"):
+        chunked.map_genome_chunks(chunks, reads, 10)
