@@ -600,7 +600,8 @@ void bits_trace_kernel(BandArgs a)
             continue;
         }
         const int i = row - col - dlo;
-        const int g = (col - 1) >> 5, j0 = (col - 1) & 31;
+        const int g = (col - 1) >> 5;
+        int j0 = (col - 1) & 31;
         if (i != ci || g != cg) {
             // D words of blocks 4g .. 4g+3 (clamped to the read's last block: bits of
             // columns past the walk's cell are never looked at)
@@ -632,8 +633,8 @@ void bits_trace_kernel(BandArgs a)
         // and differing characters
         const uint32_t below = (2u << j0) - 1u;   // bits 0 .. j0 (j0 = 31: all)
         const uint32_t fail = dseg & mseg & below;
-        int rr = fail ? j0 - (31 - __builtin_clz(fail)) : j0 + 1;
-        rr = imin(rr, imin(row, cap - n));
+        const int rf = fail ? j0 - (31 - __builtin_clz(fail)) : j0 + 1;
+        const int rr = imin(rf, imin(row, cap - n));
         if (rr > 0) {
             // rr diagonal steps: 0 bits in the transcript; v drops by the mismatches
             const uint32_t run = below & ~((1u << (j0 + 1 - rr)) - 1u);
@@ -647,7 +648,10 @@ void bits_trace_kernel(BandArgs a)
             n = n2;
             row -= rr;
             col -= rr;
-            continue;
+            // a run that ends on its segment's failing cell takes that cell's step
+            // now (the loop's next pass would come back here with the same words)
+            if (!fail || rr != rf || (stop_row0 && row == 0)) continue;
+            j0 -= rr;
         }
         // one cell that is not on a diagonal run (col >= 1)
         const int q = (col - 1) >> 3, jb = (col - 1) & 7;
