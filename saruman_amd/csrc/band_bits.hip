@@ -38,6 +38,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "nw_device.h"
 #include "nw_seq.h"
 
@@ -277,12 +279,16 @@ void bits_band_kernel(BandArgs a)
                 __builtin_nontemporal_store(rbuf, &rrow[((cb - 1) >> 4) * 64]);
             }
             store_boundary(q);   // column cb - 1 = 8q (garbage on lanes past their read)
+            // every lane steps every column of the tile (no branch per column); a
+            // lane past its read keeps the deltas of its column L - 1.  Blocks where
+            // every lane with a candidate is inside its read (most of them: a tile's
+            // reads are of about one length) run a copy of the steps without that
+            // select
+            auto block_columns = [&](auto every_live) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                // every lane steps every column of the tile (no branch per column);
-                // a lane past its read keeps the deltas of its column L - 1
                 const int c = cb + j;
-                const bool live = c < L;
+                const bool live = decltype(every_live)::value || c < L;
                 {
                     const int o = c - 1;   // code offset of the column
                     const uint32_t sh = (uint32_t)(o & 31);
@@ -346,6 +352,11 @@ void bits_band_kernel(BandArgs a)
                     }
                 }
             }
+            };
+            if (__all(!active || cb + 8 <= L))
+                block_columns(std::true_type{});
+            else
+                block_columns(std::false_type{});
             // loads for the next block's window refill (columns from cb + 8), issued
             // before this block's stores
             {

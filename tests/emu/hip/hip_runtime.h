@@ -43,4 +43,7 @@ inline int __builtin_amdgcn_sbfe(int x, uint32_t off, uint32_t w)
     return (int)((int64_t)((uint64_t)(uint32_t)x << (64 - off - w)) >> (64 - w));
 }
 inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t x) { return x; }
+// a wave vote, taken here over the one lane that runs: a kernel may use it only for
+// choices that are correct for each lane that votes true
+inline bool __all(bool x) { return x; }
 inline int __builtin_amdgcn_readfirstlane(int x) { return x; }
