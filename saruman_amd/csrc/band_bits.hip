@@ -545,6 +545,7 @@ void bits_trace_kernel(BandArgs a)
     // step and waits for memory once per segment
     int ci = -1, cg = -1;
     uint32_t dseg = 0, mseg = 0;
+    uint32_t r0 = 0, r1 = 0;   // read codes of segment cg (rows 2cg, 2cg + 1)
     const int qlast = (L - 2) >> 3;   // last block holding columns of this read
     // the side cells of a non-diagonal step (block sq of diagonal si): D0 bits of
     // diagonals si - 1 and si + 1 and the block's boundary
@@ -625,7 +626,10 @@ void bits_trace_kernel(BandArgs a)
             // characters: read codes 32g .. 32g+31 (rows 2g, 2g+1) against window
             // codes from band cell 0's code + i + 32g (exact for rows and columns
             // >= 1, the only cells whose characters can decide)
-            const uint32_t r0 = rrow[(2 * g) * 64], r1 = rrow[(2 * g + 1) * 64];
+            if (g != cg) {   // a new diagonal in the same segment keeps its read codes
+                r0 = rrow[(2 * g) * 64];
+                r1 = rrow[(2 * g + 1) * 64];
+            }
             const int wp = imax(i, 0) + 32 * g;
             const uint32_t *__restrict__ ww = wrow + (size_t)(wp >> 4) * 64;
             const uint32_t g0 = ww[0], g1 = ww[64], g2 = ww[128];
@@ -670,11 +674,13 @@ void bits_trace_kernel(BandArgs a)
             const uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
             dm = dbyte(blk, i - 1);
             dp = dbyte(blk, i + 1);
-            bvt = (int)blk[DW * 64];
+            if (q != sq) {   // the boundary belongs to the block, not the diagonal
+                bvt = (int)blk[DW * 64];
 #pragma unroll
-            for (int m = 0; m < NWD; ++m) {
-                bvp[m] = blk[(DW + 1 + m) * 64];
-                bvn[m] = blk[(DW + 1 + NWD + m) * 64];
+                for (int m = 0; m < NWD; ++m) {
+                    bvp[m] = blk[(DW + 1 + m) * 64];
+                    bvn[m] = blk[(DW + 1 + NWD + m) * 64];
+                }
             }
             si = i;
             sq = q;
