@@ -546,6 +546,8 @@ void bits_trace_kernel(BandArgs a)
     int ci = -1, cg = -1;
     uint32_t dseg = 0, mseg = 0;
     uint32_t r0 = 0, r1 = 0;   // read codes of segment cg (rows 2cg, 2cg + 1)
+    int wl0 = -1;              // window row of wg0 (wg0 .. wg2: three window rows from it)
+    uint32_t wg0 = 0, wg1 = 0, wg2 = 0;
     const int qlast = (L - 2) >> 3;   // last block holding columns of this read
     // the side cells of a non-diagonal step (block sq of diagonal si): D0 bits of
     // diagonals si - 1 and si + 1 and the block's boundary
@@ -631,11 +633,16 @@ void bits_trace_kernel(BandArgs a)
                 r1 = rrow[(2 * g + 1) * 64];
             }
             const int wp = imax(i, 0) + 32 * g;
-            const uint32_t *__restrict__ ww = wrow + (size_t)(wp >> 4) * 64;
-            const uint32_t g0 = ww[0], g1 = ww[64], g2 = ww[128];
+            if ((wp >> 4) != wl0) {   // a diagonal next to the last one mostly shares them
+                const uint32_t *__restrict__ ww = wrow + (size_t)(wp >> 4) * 64;
+                wg0 = ww[0];
+                wg1 = ww[64];
+                wg2 = ww[128];
+                wl0 = wp >> 4;
+            }
             const uint32_t gsh = (uint32_t)(wp & 15) * 2u;
-            const uint32_t x0 = r0 ^ __builtin_amdgcn_alignbit(g1, g0, gsh);
-            const uint32_t x1 = r1 ^ __builtin_amdgcn_alignbit(g2, g1, gsh);
+            const uint32_t x0 = r0 ^ __builtin_amdgcn_alignbit(wg1, wg0, gsh);
+            const uint32_t x1 = r1 ^ __builtin_amdgcn_alignbit(wg2, wg1, gsh);
             mseg = even_bits(x0 | (x0 >> 1)) | (even_bits(x1 | (x1 >> 1)) << 16);
             dseg = 0u;
 #pragma unroll
