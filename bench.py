@@ -36,7 +36,7 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 # Source: profiles/bits_band_and_trace_kernels_mixed_2M_reads.md
 PMC_TRAFFIC = [
     {"workload": "mixed", "candidates": 1774238, "tuning": "defaults",
-     "kernel_sha": "d41902831d6f", "bytes": 4553348315},
+     "kernel_sha": "144bad373c94", "bytes": 4344380891},
 ]
 
 
