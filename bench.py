@@ -36,7 +36,7 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 # Source: profiles/bits_band_and_trace_kernels_mixed_2M_reads.md
 PMC_TRAFFIC = [
     {"workload": "mixed", "candidates": 1774238, "tuning": "defaults",
-     "kernel_sha": "144bad373c94", "bytes": 4344380891},
+     "kernel_sha": "144bad373c94", "bytes": 4343037562},
 ]
 
 
@@ -230,8 +230,8 @@ def device_filter_leg(ctx, index, genome, reads, chunk, host_strands):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed", choices=sorted(WORKLOAD_TEXT))
     ap.add_argument("--reads", type=int, default=None, help="override the read count")
     ap.add_argument("--plain-steps", type=int, default=1)
