@@ -184,6 +184,106 @@ def prepare_batch(strand: Strand, genome: bytes, reads: list, L: int, k: int, do
     return out
 
 
+def seashore_read(strand: Strand, data: list, offs: list, counter: int, read_id: int,
+                  tmp_read: bytes, genome: bytes, P: Params, do_reverse: int, base: int = 0):
+    """run_seashore's body for one read, hash.c:502-834.
+
+    data: the chunk's data_array (0-terminated position lists, hash.c:396-461);
+    offs: data_array_offsets; counter: index of the read's first list in offs;
+    tmp_read: the read as the strand sees it (reverse complement for strand 4).
+    Adds the read's candidates to `strand` and returns the S of every perfect hit
+    the reference prints (hash.c:564-583), in order.
+    """
+    L, k, q, rest = P.L, P.k, P.q, P.rest
+    half = P.n // 2
+    dr = do_reverse
+    printed = []
+
+    def D(j):
+        # reads past the end of the array see zeros (calloc'd / memset tail, hash.c:384-461)
+        return data[j] if 0 <= j < len(data) else 0
+
+    def Db(list_start, idx):
+        # first-pass read of list b at index idx (Q12); counts reads that land
+        # outside list b
+        v = D(list_start + idx)
+        end = list_start
+        while end < len(data) and data[end] != 0:
+            end += 1
+        if list_start + idx > end:
+            STATS["spill"] += 1
+        return v
+
+    for a in range(half):
+        fqm = 1 if a == 0 else 0
+        in_read = a * q + 1
+        off = 0
+        Pa = D(offs[counter + a] + off)
+        while Pa > 0:
+            gs = Pa - in_read
+            if gs + 1 - base > 0:
+                tg = genome[gs + 1 - base:gs + 1 - base + L]
+                tg = tg.split(b"\x00")[0]
+                if tg == tmp_read:
+                    if strand.add_start_position(read_id, -(gs + 1), 0, P.exact_tresh,
+                                                 P.exact_tresh, k) != 1:
+                        printed.append(gs + 1)
+                possible = 1
+                # first pass, hash.c:606-692
+                for b in range(a + 1, half):
+                    rstart = b * q + 1
+                    for err in range(-k, k + 1):
+                        Pb = Db(offs[counter + b], off)
+                        nxt = 0
+                        while Pb > 0:
+                            if Pb == gs + rstart + err:
+                                if b == half - 1 and fqm:
+                                    flag, pos = 1 + dr, gs + 1
+                                elif a == half - 1:
+                                    if Pa + q > L:
+                                        flag, pos = 3 + dr, gs - k + 1
+                                    else:
+                                        flag, pos = 2 + dr, gs - k + 1
+                                elif fqm:
+                                    flag, pos = 1 + dr, gs + 1
+                                else:
+                                    flag, pos = 3 + dr, gs - k + 1
+                                possible += 1
+                                strand.add_start_position(read_id, pos, flag, possible,
+                                                          P.exact_tresh, k)
+                            nxt += 1
+                            Pb = D(offs[counter + b] + nxt)
+                # second pass, hash.c:695-794
+                for j in range(half):
+                    lim = abs(half - 1 - j - a)
+                    for err in range(-lim, lim + 1):
+                        aerr = -err if j - a < 0 else err
+                        nin = (half - j - 1) * q + 1 + rest
+                        so = 0
+                        Pb = D(offs[counter + half + j] + so)
+                        while Pb > 0:
+                            if Pb == gs + nin + aerr:
+                                if j == 0 and fqm:
+                                    flag, pos = 1 + dr, gs + 1
+                                elif j == 0:
+                                    if nin + q > L:
+                                        flag, pos = 3 + dr, gs - k + 1
+                                    else:
+                                        flag, pos = 2 + dr, gs - k + 1
+                                elif fqm:
+                                    flag, pos = 1 + dr, gs + 1
+                                else:
+                                    flag, pos = 3 + dr, gs - k + 1
+                                possible += 1
+                                strand.add_start_position(read_id, pos, flag, possible,
+                                                          P.exact_tresh, k)
+                            so += 1
+                            Pb = D(offs[counter + half + j] + so)
+            off += 1
+            Pa = D(offs[counter + a] + off)
+    return printed
+
+
 def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_reverse: int,
                chunk_value: int, index: dict | None = None, base: int = 0,
                limit: int | None = None, window_genome: bytes | None = None):
@@ -194,8 +294,7 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
     lines printed by the filter (hash.c:580).
     """
     P = derive_params(L, k)
-    q, n, rest = P.q, P.n, P.rest
-    half = n // 2
+    q, n = P.q, P.n
     if index is None:
         index = build_index(genome, q, base)
     total_bases = len(genome) if limit is None else limit
@@ -205,7 +304,6 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
     read_number = len(reads)
     chunksize = chunk_value
     max_runs = int(math.ceil(read_number / float(chunksize)))
-    dr = do_reverse
 
     for run in range(max_runs):
         # ---- data_array construction, hash.c:377-461 ----
@@ -223,20 +321,6 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
                     data.extend(index.get(qg, []))
                     data.append(0)
 
-        def D(j):
-            return data[j] if 0 <= j < len(data) else 0
-
-        def Db(list_start, idx, list_index):
-            # first-pass read of list b at index idx (Q12); counts reads that land
-            # outside list b
-            v = D(list_start + idx)
-            end = list_start
-            while end < len(data) and data[end] != 0:
-                end += 1
-            if list_start + idx > end:
-                STATS["spill"] += 1
-            return v
-
         # ---- run_seashore, hash.c:473-839 ----
         limit = read_number
         if (run + 1) * chunksize > limit:
@@ -247,77 +331,12 @@ def run_strand(genome: bytes, reads: list, names: list, L: int, k: int, do_rever
         while counter2 < limit and read_id < (run + 1) * chunksize:
             fwd_read = reads[read_id]
             tmp_read = reverse_complement(fwd_read) if do_reverse == 4 else fwd_read
-            for a in range(half):
-                fqm = 1 if a == 0 else 0
-                in_read = a * q + 1
-                off = 0
-                Pa = D(offs[counter + a] + off)
-                while Pa > 0:
-                    gs = Pa - in_read
-                    if gs + 1 - base > 0:
-                        tg = genome[gs + 1 - base:gs + 1 - base + L]
-                        tg = tg.split(b"\x00")[0]
-                        if tg == tmp_read:
-                            if strand.add_start_position(read_id, -(gs + 1), 0, P.exact_tresh,
-                                                         P.exact_tresh, k) != 1:
-                                perfect.append(
-                                    "%s\t%d\t%d\t%s\t%s\t%s\t%d\n" % (
-                                        names[read_id], gs + 1, gs + L,
-                                        "<<" if do_reverse else ">>",
-                                        tg.decode(), tmp_read.decode(), 0))
-                        possible = 1
-                        # first pass, hash.c:606-692
-                        for b in range(a + 1, half):
-                            rstart = b * q + 1
-                            for err in range(-k, k + 1):
-                                Pb = Db(offs[counter + b], off, b)
-                                nxt = 0
-                                while Pb > 0:
-                                    if Pb == gs + rstart + err:
-                                        if b == half - 1 and fqm:
-                                            flag, pos = 1 + dr, gs + 1
-                                        elif a == half - 1:
-                                            if Pa + q > L:
-                                                flag, pos = 3 + dr, gs - k + 1
-                                            else:
-                                                flag, pos = 2 + dr, gs - k + 1
-                                        elif fqm:
-                                            flag, pos = 1 + dr, gs + 1
-                                        else:
-                                            flag, pos = 3 + dr, gs - k + 1
-                                        possible += 1
-                                        strand.add_start_position(read_id, pos, flag, possible,
-                                                                  P.exact_tresh, k)
-                                    nxt += 1
-                                    Pb = D(offs[counter + b] + nxt)
-                        # second pass, hash.c:695-794
-                        for j in range(half):
-                            lim = abs(half - 1 - j - a)
-                            for err in range(-lim, lim + 1):
-                                aerr = -err if j - a < 0 else err
-                                nin = (half - j - 1) * q + 1 + rest
-                                so = 0
-                                Pb = D(offs[counter + half + j] + so)
-                                while Pb > 0:
-                                    if Pb == gs + nin + aerr:
-                                        if j == 0 and fqm:
-                                            flag, pos = 1 + dr, gs + 1
-                                        elif j == 0:
-                                            if nin + q > L:
-                                                flag, pos = 3 + dr, gs - k + 1
-                                            else:
-                                                flag, pos = 2 + dr, gs - k + 1
-                                        elif fqm:
-                                            flag, pos = 1 + dr, gs + 1
-                                        else:
-                                            flag, pos = 3 + dr, gs - k + 1
-                                        possible += 1
-                                        strand.add_start_position(read_id, pos, flag, possible,
-                                                                  P.exact_tresh, k)
-                                    so += 1
-                                    Pb = D(offs[counter + half + j] + so)
-                    off += 1
-                    Pa = D(offs[counter + a] + off)
+            for S in seashore_read(strand, data, offs, counter, read_id, tmp_read, genome, P,
+                                   do_reverse, base):
+                perfect.append(
+                    "%s\t%d\t%d\t%s\t%s\t%s\t%d\n" % (
+                        names[read_id], S, S + L - 1, "<<" if do_reverse else ">>",
+                        tmp_read.decode(), tmp_read.decode(), 0))
             # cuda_queue_check, hash.c:835, 841-851
             if strand.queue > chunk_value:
                 b = prepare_batch(strand, genome, reads, L, k, do_reverse, total_bases, base,

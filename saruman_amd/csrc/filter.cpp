@@ -225,8 +225,11 @@ int add_start(ReadOut &ro, int32_t pos, int flag, int possible, int exact_tresh,
 }
 
 // run_seashore for one read (hash.c:502-834).  genome/glen: the text the perfect
-// check reads, whose offset 0 is genome position `base` (a -G chunk, hash.c:545-552)
-void seashore_read(Lists &ls, size_t rid, const Params &P, const char *genome, size_t glen,
+// check reads, whose offset 0 is genome position `base` (a -G chunk, hash.c:545-552).
+// LS gives the chunk's data_array: reads, roff and D(read, list, index) as Lists has
+// them.
+template <class LS>
+void seashore_read(LS &ls, size_t rid, const Params &P, const char *genome, size_t glen,
                    long base, int do_reverse, ReadOut &ro)
 {
     const int q = P.q, half = P.half, k = P.k, L = P.L, rest = P.rest, dr = do_reverse;
