@@ -89,6 +89,23 @@ typedef struct srm_ut_hash_handle {
     unsigned hashv;
 } srm_ut_hash_handle;
 
+/* uthash 1.9.4 bucket and the head of its table (src/uthash.h:836-865), as far as
+ * the bucket lookup of a read id reads them (HASH_FIND_INT, src/uthash.h:84-95,
+ * 568-579): the bucket array, its size and the handle's offset in an element */
+typedef struct srm_ut_hash_bucket {
+    srm_ut_hash_handle *hh_head;
+    unsigned count;
+    unsigned expand_mult;
+} srm_ut_hash_bucket;
+
+typedef struct srm_ut_hash_table {
+    srm_ut_hash_bucket *buckets;
+    unsigned num_buckets, log2_num_buckets;
+    unsigned num_items;
+    srm_ut_hash_handle *tail;
+    ptrdiff_t hho;
+} srm_ut_hash_table;
+
 /* src/util.h:62-66 */
 struct read_name_hash {
     int read_number;
@@ -137,6 +154,7 @@ extern float alignment_time[2];                    /* cuda-functions.cu:61 (ms, 
 #define SRM_MAX_W 900
 #define SRM_BAND_MAX 64      /* widest band the banded kernel holds in registers */
 #define SRM_BAND_KMAX 14     /* saturated band values must fit a nibble: k + 1 <= 15 */
+#define SRM_PLAIN_CHUNK ((size_t)64 << 20)   /* plain-matrix chunk under a pool cap (bytes) */
 
 /*
  * One candidate alignment.  The read is given in the orientation it is aligned
@@ -193,6 +211,14 @@ static inline uint32_t srm_ops_words(uint32_t w, uint32_t k) { return (w + k + 1
 /* context = one HIP stream + device pool on one GPU */
 int srm_ctx_create(int device, size_t pool_cap_bytes, srm_ctx **out);
 void srm_ctx_destroy(srm_ctx *ctx);
+/* device bytes the context's buffers (its genome and its jobs') hold now */
+size_t srm_ctx_pool_used(const srm_ctx *ctx);
+/* bound on those bytes for later allocations (0 = none): a buffer that would
+ * take the pool past it fails with SRM_ENOMEM */
+int srm_ctx_set_pool_cap(srm_ctx *ctx, size_t pool_cap_bytes);
+/* testing hook for the failure paths: the next `count` device allocations of
+ * any context fail with SRM_ENOMEM (0 clears) */
+void srm_debug_fail_allocs(int count);
 
 /*
  * Keep a packed genome resident on the device (uploaded once).  words/n_words
@@ -230,6 +256,20 @@ int srm_band_occupancy(int nw, int unit);
  */
 int srm_job_create(srm_ctx *ctx, srm_job **out);
 void srm_job_destroy(srm_job *job);
+/* frees the job's device buffers (after waiting for its work); the next upload
+ * allocates them again */
+int srm_job_release(srm_job *job);
+/* Upper bound on the device bytes a job of ctx (NULL: default tuning) needs for
+ * these candidates: its uploads, band scratch, results and transcripts, a chunk of
+ * plain matrices (SRM_PLAIN_CHUNK, or one matrix if larger) when any candidate runs
+ * on the plain kernel, and with_records the record stage (srm_job_set_records /
+ * srm_job_records with a name table or prefix of names_bytes bytes, names printed
+ * with at most name_max characters).  A context capped at what it already holds
+ * plus this bound runs the job. */
+size_t srm_job_bytes_bound(const srm_ctx *ctx, int match, int mismatch, const srm_cand *cand, int n,
+                           size_t n_read_words, size_t n_win_words, size_t n_read_bytes,
+                           size_t n_win_bytes, int kernel, int with_records, size_t names_bytes,
+                           int name_max);
 int srm_job_upload(srm_job *job, int match, int mismatch, const srm_cand *cand, int n,
                    const uint32_t *read_words, size_t n_read_words, const uint32_t *win_words,
                    size_t n_win_words, const char *read_bytes, size_t n_read_bytes,
@@ -240,7 +280,8 @@ int srm_job_run(srm_job *job, int kernel);
 int srm_job_download(srm_job *job, srm_result *res, uint32_t *ops, size_t n_ops_words);
 /* device time (ms) of the last srm_job_run, measured with HIP events on the job's stream */
 float srm_job_last_ms(srm_job *job);
-/* per-kernel device time (ms) of the last run: kind 0 = banded, 1 = plain */
+/* device time (ms) of the last eager run by stage: kind 0 = banded, 1 = plain,
+ * 2 = records (srm_job_set_records) */
 float srm_job_kernel_ms(srm_job *job, int kind);
 /* capture srm_job_run(job, kernel) into a hipGraph and replay it; 0 = plain launches */
 int srm_job_use_graph(srm_job *job, int enable);
@@ -262,6 +303,20 @@ int srm_job_path_count(srm_job *job, int path);
 long long srm_job_records(srm_job *job, const int64_t *pos, const int32_t *read_id,
                           const char *names, int name_stride, size_t n_names,
                           const char *name_prefix, char *out, size_t outcap, long long *n_hits);
+
+/*
+ * The same records as a stage of the job's launch sequence: after this call every
+ * srm_job_run (eager or graph-replayed) also formats the records of every hit on
+ * the device, into an output sized to the longest text the candidates can give,
+ * with no host round trip between the alignment and the text.  The arguments are
+ * those of srm_job_records.  A new upload (or srm_job_records) removes the stage.
+ */
+int srm_job_set_records(srm_job *job, const int64_t *pos, const int32_t *read_id,
+                        const char *names, int name_stride, size_t n_names,
+                        const char *name_prefix);
+/* byte count of the last run's record text (and its number of records); copies
+ * the text to out when outcap >= that count.  Synchronous on the job's stream. */
+long long srm_job_records_text(srm_job *job, char *out, size_t outcap, long long *n_hits);
 
 /*
  * Record text for one hit: the post-processing of src/cuda-functions.cu:828-994
@@ -424,7 +479,8 @@ typedef struct srm_server srm_server;
 typedef struct srm_server_config {
     int device;
     int n_slots;          /* requests in flight on the device at once (1..16) */
-    size_t hbm_budget;    /* device bytes the in-flight requests may use (0 = 32 GiB) */
+    size_t hbm_budget;    /* device bytes the alignment slots may hold, resident genome
+                             and kept buffers included (0 = 32 GiB) */
     int filter_threads;   /* host threads of the q-gram filter (0 = all) */
     int format_threads;   /* host threads of record formatting (0 = all) */
     int chunk_value;      /* the reference's -s for every request */
@@ -439,6 +495,7 @@ typedef struct srm_request_stats {
     long long n_reads, n_candidates, n_hits, n_perfect, n_mapped;
     long long dev_bytes;   /* device footprint charged against the HBM budget */
     int slot, status;
+    char message[160];     /* what failed, when status != SRM_OK */
 } srm_request_stats;
 
 int srm_server_create(const srm_server_config *cfg, const char *genome, size_t glen,
@@ -453,6 +510,10 @@ long long srm_server_result(srm_server *s, long long id, char *out, size_t outca
 void srm_server_hold(srm_server *s, int on);
 /* queue depths: not yet filtered, waiting for admission, on a slot */
 void srm_server_queue(srm_server *s, uint64_t *incoming, uint64_t *ready, uint64_t *running);
+/* HBM of the alignment slots: what they account for now (idle slots: the buffers
+ * they keep; busy slots: their reservation, enforced as their pool cap) and the
+ * most accounted at once.  Both stay within hbm_budget. */
+void srm_server_memory(srm_server *s, size_t *now_bytes, size_t *peak_bytes);
 void srm_server_destroy(srm_server *s);
 
 #ifdef __cplusplus

@@ -38,9 +38,16 @@ _PROTOS = [
     ("srm_device_sync", C.c_int, [C.c_int]),
     ("srm_ctx_create", C.c_int, [C.c_int, C.c_size_t, C.POINTER(_vp)]),
     ("srm_ctx_destroy", None, [_vp]),
+    ("srm_ctx_pool_used", C.c_size_t, [_vp]),
+    ("srm_ctx_set_pool_cap", C.c_int, [_vp, C.c_size_t]),
+    ("srm_debug_fail_allocs", None, [C.c_int]),
     ("srm_ctx_set_genome", C.c_int, [_vp, _vp, C.c_size_t]),
     ("srm_job_create", C.c_int, [_vp, C.POINTER(_vp)]),
     ("srm_job_destroy", None, [_vp]),
+    ("srm_job_release", C.c_int, [_vp]),
+    ("srm_job_bytes_bound", C.c_size_t, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_size_t,
+                                         C.c_size_t, C.c_size_t, C.c_size_t, C.c_int, C.c_int,
+                                         C.c_size_t, C.c_int]),
     ("srm_job_upload", C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_size_t, _vp,
                                  C.c_size_t, _vp, C.c_size_t, _vp, C.c_size_t]),
     ("srm_job_run", C.c_int, [_vp, C.c_int]),
@@ -51,6 +58,8 @@ _PROTOS = [
     ("srm_job_path_count", C.c_int, [_vp, C.c_int]),
     ("srm_job_records", C.c_longlong, [_vp, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_char_p, _vp,
                                        C.c_size_t, _vp]),
+    ("srm_job_set_records", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_char_p]),
+    ("srm_job_records_text", C.c_longlong, [_vp, _vp, C.c_size_t, _vp]),
     ("srm_format_record", C.c_int, [C.c_char_p, C.c_uint, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _vp, _vp, _vp, _vp, C.c_char_p, C.c_int]),
     ("srm_slot_rows", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
@@ -108,6 +117,7 @@ _PROTOS = [
     ("srm_server_result", C.c_longlong, [_vp, C.c_longlong, _vp, C.c_size_t, _vp, C.c_int]),
     ("srm_server_hold", None, [_vp, C.c_int]),
     ("srm_server_queue", None, [_vp, _vp, _vp, _vp]),
+    ("srm_server_memory", None, [_vp, _vp, _vp]),
     ("srm_server_destroy", None, [_vp]),
     # drop-in entry points (src/cuda-functions.h:55-78)
     ("cuda_needleman_wunsch", None, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp,

@@ -218,6 +218,31 @@ class Job:
                 check(int(got), "srm_job_records")
         return (out[:got].tobytes() if copy else int(got)), int(hits.value)
 
+    def set_records(self, pos, read_id=None, names=None, name_stride=60, prefix=""):
+        """Makes record formatting on the device a stage of every later run
+        (srm_job_set_records); arguments as for records()."""
+        pos = np.ascontiguousarray(pos, np.int64)
+        rid = None if read_id is None else np.ascontiguousarray(read_id, np.int32)
+        nb = None if names is None else np.frombuffer(names, np.uint8)
+        n_names = 0 if nb is None else nb.size // name_stride
+        self._rec_keep = (pos, rid, nb)
+        check(self.lib.srm_job_set_records(self.h, ptr(pos), ptr(rid), ptr(nb), name_stride,
+                                           n_names, prefix.encode()), "srm_job_set_records")
+
+    def records_text(self, out=None, copy=True):
+        """The last run's record text (srm_job_records_text): (bytes, hits), or
+        (byte count, hits) with copy=False and the text in out."""
+        hits = C.c_longlong()
+        total = self.lib.srm_job_records_text(self.h, None, 0, C.byref(hits))
+        if total < 0:
+            check(int(total), "srm_job_records_text")
+        if out is None or out.size < total:
+            out = np.empty(max(int(total), 1), np.uint8)
+        got = self.lib.srm_job_records_text(self.h, ptr(out), out.size, C.byref(hits))
+        if got < 0:
+            check(int(got), "srm_job_records_text")
+        return (out[:got].tobytes() if copy else int(got)), int(hits.value)
+
     def close(self):
         if self.h:
             self.lib.srm_job_destroy(self.h)

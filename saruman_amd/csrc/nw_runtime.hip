@@ -68,6 +68,9 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// allocations that are made to fail (srm_debug_fail_allocs), process-wide
+std::atomic<int> g_fail_allocs{0};
+
 }  // namespace
 
 struct srm_ctx {
@@ -90,7 +93,12 @@ struct srm_ctx {
     int ensure(DevBuf &b, size_t bytes)
     {
         if (bytes <= b.cap) return SRM_OK;
-        size_t want = std::max(bytes, b.cap + b.cap / 2);
+        for (int f = g_fail_allocs.load(); f > 0;)
+            if (g_fail_allocs.compare_exchange_weak(f, f - 1))
+                return fail(SRM_ENOMEM, "injected allocation failure (%zu bytes)", bytes);
+        // under a pool cap a buffer grows to exactly its 4 KiB-rounded need (what
+        // srm_job_bytes_bound counts); otherwise by half again, for fewer regrowths
+        size_t want = pool_cap ? bytes : std::max(bytes, b.cap + b.cap / 2);
         want = (want + 4095) & ~(size_t)4095;
         if (pool_cap && pool_used - b.cap + want > pool_cap) {
             want = bytes;
@@ -152,19 +160,28 @@ struct srm_job {
     DevBuf d_rec_len, d_rec_off, d_rec_info, d_rec_tmp, d_rec_out, d_rec_pos, d_rec_id,
         d_rec_names, d_rec_hits;                    // device record formatting
     size_t ops_words = 0;
+    // record stage of the launch sequence (srm_job_set_records)
+    bool rec_stage = false;
+    bool rec_names = false, rec_ids = false;
+    int rec_stride = 0, rec_prefix_len = 0;
+    size_t rec_tmp = 0;               // scan temporary storage
+    size_t rec_cap = 0;               // output bytes: a bound on the text of every line
+    size_t rec_rows_bound = 0;        // sum over candidates of 2 (w + k): both rows at most
+    bool rec_done = false;            // a run since the stage was set up formatted records
     // host-side plan
     std::vector<BandClass> classes;
     std::vector<PlainChunk> chunks_rest, chunks_all, chunks_bytes;
     size_t n_band = 0, n_rest = 0, n_bytes = 0;
     size_t mat_budget_elems = 0;
+    size_t mat_elems_band = 0, mat_elems_plain = 0;   // matrix elements a run of each path needs
     // timing
-    hipEvent_t ev[3] = {};
+    hipEvent_t ev[4] = {};               // start, after band, after plain, after records
     hipEvent_t cs[9] = {}, ce[9] = {};   // around each band-class launch
     hipEvent_t fork_ev = nullptr;        // concurrent classes: fork and joins
     hipEvent_t join_ev[8] = {};
     std::vector<hipStream_t> side;       // streams of concurrent band classes
     int class_streams = 0;               // at the last upload
-    float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f;
+    float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f, rec_ms = 0.f;
     float class_ms[9] = {};          // by NW (1..8), last eager run
     // graph
     bool graph_enabled = false;
@@ -239,6 +256,37 @@ void plan_plain(const std::vector<uint32_t> &idx, const srm_cand *cand, size_t b
         cur.mat_elems = acc;
         chunks.push_back(cur);
     }
+}
+
+// the record kernels' arguments over the job's buffers (set_up: the record stage
+// of the launch sequence, names and ids as srm_job_set_records stored them)
+srm::RecArgs rec_args(srm_job *j, bool set_up)
+{
+    srm_ctx *ctx = j->ctx;
+    srm::RecArgs a{};
+    a.n = j->n;
+    a.cand = (const srm_cand *)j->d_cand.p;
+    a.res = (const srm_result *)j->d_res.p;
+    a.ops = (const uint32_t *)j->d_ops.p;
+    a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
+    a.win_words = j->resident_genome ? ctx->genome_ptr()
+                                     : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
+    a.read_bytes = (const char *)j->d_read_bytes.p;
+    a.win_bytes = (const char *)j->d_win_bytes.p;
+    a.pos = (const int64_t *)j->d_rec_pos.p;
+    a.len = (uint64_t *)j->d_rec_len.p;
+    a.info = (uint4 *)j->d_rec_info.p;
+    a.hits = (uint32_t *)j->d_rec_hits.p;
+    a.off = (const uint64_t *)j->d_rec_off.p;
+    a.out = (char *)j->d_rec_out.p;
+    if (set_up) {
+        a.read_id = j->rec_ids ? (const int32_t *)j->d_rec_id.p : nullptr;
+        a.names = j->rec_names ? (const char *)j->d_rec_names.p : nullptr;
+        a.name_stride = j->rec_stride;
+        a.prefix = j->rec_names ? nullptr : (const char *)j->d_rec_names.p;
+        a.prefix_len = j->rec_prefix_len;
+    }
+    return a;
 }
 
 int run_launches(srm_job *j, int kernel, bool timed)
@@ -327,6 +375,18 @@ int run_launches(srm_job *j, int kernel, bool timed)
     rc = plain(j->chunks_bytes, j->d_idx_bytes, j->d_off_bytes, false);
     if (rc) return rc;
     if (timed) HIP_TRY(hipEventRecord(j->ev[2], st));
+    if (j->rec_stage) {
+        // records of every hit into the output sized at set-up: counter reset (a
+        // kernel, so that a graph can hold it), line lengths, their scan, the text
+        srm::RecArgs a = rec_args(j, true);
+        HIP_TRY(srm::launch_reset(a.hits, 1, st));
+        HIP_TRY(srm::launch_rec_measure(a, st));
+        size_t tmp = j->rec_tmp;
+        HIP_TRY(srm::rec_scan(j->d_rec_tmp.p, tmp, a.len, (uint64_t *)j->d_rec_off.p, j->n, st));
+        HIP_TRY(srm::launch_rec_write(a, st));
+    }
+    if (timed) HIP_TRY(hipEventRecord(j->ev[3], st));
+    j->rec_done = j->rec_stage;
     return SRM_OK;
 }
 
@@ -497,13 +557,11 @@ int srm_job_create(srm_ctx *ctx, srm_job **out)
     return SRM_OK;
 }
 
-void srm_job_destroy(srm_job *j)
+namespace {
+// frees every device buffer of a job (its context's pool shrinks by as much)
+void release_job_buffers(srm_job *j)
 {
-    if (!j) return;
     srm_ctx *c = j->ctx;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    destroy_graph(j);
     DevBuf *bufs[] = {&j->d_cand, &j->d_read_words, &j->d_win_words, &j->d_read_bytes,
                       &j->d_win_bytes, &j->d_perm, &j->d_cand_tile, &j->d_tile_score,
                       &j->d_tiles, &j->d_tile_scratch,
@@ -514,17 +572,117 @@ void srm_job_destroy(srm_job *j)
                       &j->d_rec_out, &j->d_rec_pos, &j->d_rec_id, &j->d_rec_names,
                       &j->d_rec_hits};
     for (DevBuf *b : bufs) c->release(*b);
+}
+}  // namespace
+
+int srm_job_release(srm_job *j)
+{
+    srm_ctx *c = j->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (hipStream_t q : j->side) HIP_TRY(hipStreamSynchronize(q));
+    destroy_graph(j);
+    release_job_buffers(j);
+    j->uploaded = false;
+    return SRM_OK;
+}
+
+size_t srm_ctx_pool_used(const srm_ctx *c) { return c->pool_used; }
+
+int srm_ctx_set_pool_cap(srm_ctx *c, size_t bytes)
+{
+    c->pool_cap = bytes;
+    return SRM_OK;
+}
+
+void srm_debug_fail_allocs(int count) { g_fail_allocs.store(count < 0 ? 0 : count); }
+
+void srm_job_destroy(srm_job *j)
+{
+    if (!j) return;
+    srm_ctx *c = j->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    destroy_graph(j);
+    release_job_buffers(j);
     for (hipStream_t q : j->side) {
         (void)hipStreamSynchronize(q);
         (void)hipStreamDestroy(q);
     }
     for (auto *arr : {&j->ev[0], &j->cs[0], &j->ce[0], &j->join_ev[0]}) {
-        const int cnt = arr == &j->ev[0] ? 3 : arr == &j->join_ev[0] ? 8 : 9;
+        const int cnt = arr == &j->ev[0] ? 4 : arr == &j->join_ev[0] ? 8 : 9;
         for (int i = 0; i < cnt; ++i)
             if (arr[i]) (void)hipEventDestroy(arr[i]);
     }
     if (j->fork_ev) (void)hipEventDestroy(j->fork_ev);
     delete j;
+}
+
+size_t srm_job_bytes_bound(const srm_ctx *ctx, int match, int mismatch, const srm_cand *cand, int n,
+                           size_t n_read_words, size_t n_win_words, size_t n_read_bytes,
+                           size_t n_win_bytes, int kernel, int with_records, size_t names_bytes,
+                           int name_max)
+{
+    // each term is one buffer as srm_job_upload, srm_job_run and the record stage
+    // ask for it, rounded as ensure() rounds under a pool cap
+    auto r4k = [](size_t b) { return (std::max<size_t>(b, 1) + 4095) & ~(size_t)4095; };
+    if (n < 0) return 0;
+    const size_t N = (size_t)n;
+    // the upload plans the band for every eligible candidate whatever kernel runs;
+    // the kernel decides which candidates need full matrices
+    const bool band_ok = match >= 0 && mismatch >= 1;
+    const bool all_plain = kernel == SRM_KERNEL_PLAIN;
+    const bool unit = match == 0 && mismatch == 1;
+    size_t n_cls[9] = {}, maxL_cls[9] = {};
+    size_t ops = 0, rows = 0, plain_elems = 0;
+    bool any_plain = false;
+    for (int t = 0; t < n; ++t) {
+        const srm_cand &c = cand[t];
+        ops = std::max(ops, (size_t)c.ops_off + srm_ops_words(c.w, c.k));
+        rows += 2 * ((size_t)c.w + c.k);
+        const bool banded = band_ok && !(c.bits & SRM_CAND_BYTES) && c.k <= SRM_BAND_KMAX &&
+                            band_cells(c) <= SRM_BAND_MAX && (int)c.w >= (int)c.L - (int)c.k;
+        if (banded) {
+            const int merge = ctx ? std::max(1, std::min(8, ctx->tuning.merge_below)) : 1;
+            const int nw = std::min(8, std::max((band_cells(c) + 7) / 8, merge));
+            ++n_cls[nw];
+            maxL_cls[nw] = std::max<size_t>(maxL_cls[nw], c.L);
+        }
+        if (!banded || all_plain) {
+            any_plain = true;
+            plain_elems = std::max(plain_elems, (size_t)(c.w + 1) * (size_t)(c.L + 1) + 64);
+        }
+    }
+    size_t tiles = 0, scratch = 0;
+    for (int nw = 1; nw <= 8; ++nw) {
+        if (!n_cls[nw]) continue;
+        // every (class, L mod 8) run ends with at most one partial tile
+        const size_t tc = (n_cls[nw] + 63) / 64 + 8;
+        tiles += tc;
+        scratch += tc * (unit ? srm::bits_tile_words(nw, (int)maxL_cls[nw])
+                              : tile_scratch_words(nw, (int)maxL_cls[nw]));
+    }
+    size_t b = 0;
+    b += r4k(N * sizeof(srm_cand) + 64) * 2;                  // d_cand, d_cand_tile
+    b += r4k(n_read_words * 4 + 64);
+    if (n_win_words) b += r4k(n_win_words * 4 + 64);
+    b += r4k(n_read_bytes + 64) + r4k(n_win_bytes + 64);
+    b += r4k(N * 4 + 64) + r4k(std::max<size_t>(N * 4, 64));  // d_perm, d_tile_score
+    b += r4k(tiles * 16 + 64) + r4k(tiles * 8 + 64);          // d_tiles, d_tile_scratch
+    b += 3 * r4k(N * 4 + 64) + 3 * r4k(N * 8 + 64);           // plain index and offset lists
+    b += r4k(64 * 4) + r4k(std::max<size_t>(scratch * 4, 256));
+    b += r4k(std::max<size_t>(N * sizeof(srm_result), 64)) + r4k(std::max<size_t>(ops * 4, 64));
+    if (any_plain)   // a chunk of matrices under a pool cap, at least one matrix
+        b += r4k(std::max<size_t>(SRM_PLAIN_CHUNK, plain_elems * 2) + 256);
+    if (with_records) {
+        b += r4k(N * 8 + 64) + r4k(N * 4 + 64) + r4k(names_bytes + 64);
+        b += 2 * r4k((N + 1) * 8) + r4k(std::max<size_t>(N * 16, 16)) + r4k(256);
+        size_t tmp = 0;
+        (void)srm::rec_scan(nullptr, tmp, nullptr, nullptr, n, nullptr);
+        b += r4k(std::max<size_t>(tmp, 256));
+        b += r4k(rows + N * ((size_t)std::max(name_max, 0) + 11 + 11 + 2 + 6 + 7));
+    }
+    return b;
 }
 
 int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, int n,
@@ -537,6 +695,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     destroy_graph(j);
     j->uploaded = false;
+    j->rec_stage = false;   // positions and ids belong to the previous candidates
     if (n < 0) return fail(SRM_EINVAL, "negative candidate count");
     if (!win_words && !ctx->genome_ptr() && n_win_bytes == 0)
         return fail(SRM_EINVAL, "no window source: pass win_words or set a resident genome");
@@ -568,14 +727,16 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             continue;
         }
         // code arrays: pad codes readable before 0 and after the end (the caller's
-        // arrays start with SRM_CODE_PAD pad codes for uploaded arrays)
-        if ((uint64_t)c.read_off + c.L + pad > read_codes - pad)
+        // arrays start with SRM_CODE_PAD pad codes for uploaded arrays, and offsets
+        // count from the first code after them).  Written without subtraction: an
+        // array shorter than the padding must fail, not wrap.
+        if ((uint64_t)c.read_off + c.L + 2 * pad > read_codes)
             return fail(SRM_EINVAL, "candidate %d: read codes out of bounds", t);
         if (j->resident_genome) {
             // the genome carries SRM_CODE_PAD codes of padding at both ends
             if ((uint64_t)c.win_off < pad || (uint64_t)c.win_off + c.w + pad > win_codes)
                 return fail(SRM_EINVAL, "candidate %d: genome window out of bounds", t);
-        } else if ((uint64_t)c.win_off + c.w + pad > win_codes - pad) {
+        } else if ((uint64_t)c.win_off + c.w + 2 * pad > win_codes) {
             return fail(SRM_EINVAL, "candidate %d: window codes out of bounds", t);
         }
         // banded: exact-band conditions, and w >= L - k (a shorter window can never
@@ -586,6 +747,8 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         else
             rest_idx.push_back((uint32_t)t);
     }
+    j->rec_rows_bound = 0;
+    for (int t = 0; t < n; ++t) j->rec_rows_bound += 2 * ((size_t)cand[t].w + cand[t].k);
     j->n_band = band_idx.size();
     j->n_rest = rest_idx.size();
     j->n_bytes = bytes_idx.size();
@@ -709,20 +872,6 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             all_idx.push_back(t);
         }
     }
-    std::vector<uint64_t> off_rest, off_all, off_bytes;
-    size_t free_b = 0, total_b = 0;
-    (void)hipMemGetInfo(&free_b, &total_b);
-    size_t budget = std::min<size_t>(free_b / 4, (size_t)32 << 30);
-    if (ctx->pool_cap) budget = std::min(budget, ctx->pool_cap / 4);
-    budget = std::max<size_t>(budget, (size_t)64 << 20);
-    j->mat_budget_elems = budget / 2;
-    plan_plain(rest_idx, cand, j->mat_budget_elems, off_rest, j->chunks_rest);
-    plan_plain(all_idx, cand, j->mat_budget_elems, off_all, j->chunks_all);
-    plan_plain(bytes_idx, cand, j->mat_budget_elems, off_bytes, j->chunks_bytes);
-    size_t mat_elems = 0;
-    for (auto *v : {&j->chunks_rest, &j->chunks_all, &j->chunks_bytes})
-        for (auto &ch : *v) mat_elems = std::max(mat_elems, ch.mat_elems);
-
     // ---- device buffers
     int rc;
     if ((rc = upload(ctx, j->d_cand, cand, (size_t)n))) return rc;
@@ -745,14 +894,38 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     if ((rc = upload(ctx, j->d_idx_rest, rest_idx.data(), rest_idx.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_all, all_idx.data(), all_idx.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_bytes, bytes_idx.data(), bytes_idx.size()))) return rc;
-    if ((rc = upload(ctx, j->d_off_rest, off_rest.data(), off_rest.size()))) return rc;
-    if ((rc = upload(ctx, j->d_off_all, off_all.data(), off_all.size()))) return rc;
-    if ((rc = upload(ctx, j->d_off_bytes, off_bytes.data(), off_bytes.size()))) return rc;
     if ((rc = ctx->ensure(j->d_queue, 64 * sizeof(uint32_t)))) return rc;
     if ((rc = ctx->ensure(j->d_scratch, std::max<size_t>(scratch_words * 4, 256)))) return rc;
     if ((rc = ctx->ensure(j->d_res, std::max<size_t>((size_t)n * sizeof(srm_result), 64)))) return rc;
     if ((rc = ctx->ensure(j->d_ops, std::max<size_t>(ops_end * 4, 64)))) return rc;
-    if (mat_elems && (rc = ctx->ensure(j->d_mat, mat_elems * 2 + 256))) return rc;
+
+    // ---- plain plans: full matrices in chunks of at most a quarter of free HBM (32
+    // GB at most) -- or, under a pool cap, of SRM_PLAIN_CHUNK bytes, the room
+    // srm_job_bytes_bound reserves.  A chunk holds at least one candidate.  The
+    // matrix buffer itself is allocated by srm_job_run, for the path it runs: a
+    // banded run whose candidates all fit the band needs none.
+    std::vector<uint64_t> off_rest, off_all, off_bytes;
+    size_t budget = SRM_PLAIN_CHUNK;
+    if (!ctx->pool_cap) {
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        budget = std::max<size_t>(std::min<size_t>(free_b / 4, (size_t)32 << 30), SRM_PLAIN_CHUNK);
+    }
+    j->mat_budget_elems = budget / 2;
+    plan_plain(rest_idx, cand, j->mat_budget_elems, off_rest, j->chunks_rest);
+    plan_plain(all_idx, cand, j->mat_budget_elems, off_all, j->chunks_all);
+    plan_plain(bytes_idx, cand, j->mat_budget_elems, off_bytes, j->chunks_bytes);
+    auto max_chunk = [](std::initializer_list<const std::vector<PlainChunk> *> vs) {
+        size_t m = 0;
+        for (auto *v : vs)
+            for (auto &ch : *v) m = std::max(m, ch.mat_elems);
+        return m;
+    };
+    j->mat_elems_band = max_chunk({&j->chunks_rest, &j->chunks_bytes});
+    j->mat_elems_plain = max_chunk({&j->chunks_all, &j->chunks_bytes});
+    if ((rc = upload(ctx, j->d_off_rest, off_rest.data(), off_rest.size()))) return rc;
+    if ((rc = upload(ctx, j->d_off_all, off_all.data(), off_all.size()))) return rc;
+    if ((rc = upload(ctx, j->d_off_bytes, off_bytes.data(), off_bytes.size()))) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     j->uploaded = true;
     return SRM_OK;
@@ -763,6 +936,15 @@ int srm_job_run(srm_job *j, int kernel)
     if (!j->uploaded) return fail(SRM_ESTATE, "srm_job_run before a successful upload");
     srm_ctx *ctx = j->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
+    // the plain kernel's matrices for the candidates this run sends there (before
+    // any capture: a graph holds the buffer's address)
+    const size_t mat = kernel == SRM_KERNEL_PLAIN ? j->mat_elems_plain : j->mat_elems_band;
+    if (mat) {
+        const void *before = j->d_mat.p;
+        int rc = ctx->ensure(j->d_mat, mat * 2 + 256);
+        if (rc) return rc;
+        if (j->d_mat.p != before) destroy_graph(j);
+    }
     if (j->graph_enabled) {
         if (!j->graph_exec || j->graph_kernel != kernel) {
             destroy_graph(j);
@@ -781,18 +963,19 @@ int srm_job_run(srm_job *j, int kernel)
         }
         HIP_TRY(hipEventRecord(j->ev[0], ctx->stream));
         HIP_TRY(hipGraphLaunch(j->graph_exec, ctx->stream));
-        HIP_TRY(hipEventRecord(j->ev[2], ctx->stream));
-        HIP_TRY(hipEventSynchronize(j->ev[2]));
-        HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[2]));
-        j->band_ms = j->plain_ms = -1.f;
+        HIP_TRY(hipEventRecord(j->ev[3], ctx->stream));
+        HIP_TRY(hipEventSynchronize(j->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[3]));
+        j->band_ms = j->plain_ms = j->rec_ms = -1.f;
         return SRM_OK;
     }
     int rc = run_launches(j, kernel, true);
     if (rc) return rc;
-    HIP_TRY(hipEventSynchronize(j->ev[2]));
-    HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[2]));
+    HIP_TRY(hipEventSynchronize(j->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&j->last_ms, j->ev[0], j->ev[3]));
     HIP_TRY(hipEventElapsedTime(&j->band_ms, j->ev[0], j->ev[1]));
     HIP_TRY(hipEventElapsedTime(&j->plain_ms, j->ev[1], j->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&j->rec_ms, j->ev[2], j->ev[3]));
     for (auto &x : j->class_ms) x = 0.f;
     if (kernel != SRM_KERNEL_PLAIN)   // the class events are recorded only with band launches
         for (size_t c = 0; c < j->classes.size(); ++c)
@@ -815,12 +998,17 @@ int srm_job_download(srm_job *j, srm_result *res, uint32_t *ops, size_t n_ops_wo
     return SRM_OK;
 }
 
-long long srm_job_records(srm_job *j, const int64_t *pos, const int32_t *read_id,
-                          const char *names, int name_stride, size_t n_names,
-                          const char *name_prefix, char *out, size_t outcap, long long *n_hits)
+}  // extern "C"
+
+namespace {
+// validates and uploads what the record kernels read besides the run's results:
+// window positions, read ids and the name table (or prefix); sizes the line
+// buffers and the scan's temporary storage
+int rec_inputs(srm_job *j, const int64_t *pos, const int32_t *read_id, const char *names,
+               int name_stride, size_t n_names, const char *name_prefix)
 {
-    if (!j->uploaded) return fail(SRM_ESTATE, "srm_job_records before a successful upload");
-    if (!pos) return fail(SRM_EINVAL, "srm_job_records needs the window positions");
+    if (!j->uploaded) return fail(SRM_ESTATE, "records before a successful upload");
+    if (!pos) return fail(SRM_EINVAL, "records need the window positions");
     srm_ctx *ctx = j->ctx;
     const int n = j->n;
     if (names) {
@@ -850,31 +1038,39 @@ long long srm_job_records(srm_job *j, const int64_t *pos, const int32_t *read_id
     if ((rc = ctx->ensure(j->d_rec_off, ((size_t)n + 1) * 8))) return rc;
     if ((rc = ctx->ensure(j->d_rec_info, std::max<size_t>((size_t)n * 16, 16)))) return rc;
     if ((rc = ctx->ensure(j->d_rec_hits, 256))) return rc;
-    srm::RecArgs a{};
-    a.n = n;
-    a.cand = (const srm_cand *)j->d_cand.p;
-    a.res = (const srm_result *)j->d_res.p;
-    a.ops = (const uint32_t *)j->d_ops.p;
-    a.read_words = (const uint32_t *)j->d_read_words.p + SRM_CODE_PAD / 16;
-    a.win_words = j->resident_genome ? ctx->genome_ptr()
-                                     : (const uint32_t *)j->d_win_words.p + SRM_CODE_PAD / 16;
-    a.read_bytes = (const char *)j->d_read_bytes.p;
-    a.win_bytes = (const char *)j->d_win_bytes.p;
-    a.pos = (const int64_t *)j->d_rec_pos.p;
-    a.read_id = read_id ? (const int32_t *)j->d_rec_id.p : nullptr;
-    a.names = names ? (const char *)j->d_rec_names.p : nullptr;
-    a.name_stride = name_stride;
-    a.prefix = names ? nullptr : (const char *)j->d_rec_names.p;
-    a.prefix_len = (int)plen;
-    a.len = (uint64_t *)j->d_rec_len.p;
-    a.info = (uint4 *)j->d_rec_info.p;
-    a.hits = (uint32_t *)j->d_rec_hits.p;
-    a.off = (const uint64_t *)j->d_rec_off.p;
+    size_t tmp = 0;
+    HIP_TRY(srm::rec_scan(nullptr, tmp, (const uint64_t *)j->d_rec_len.p,
+                          (uint64_t *)j->d_rec_off.p, n, ctx->stream));
+    if ((rc = ctx->ensure(j->d_rec_tmp, std::max<size_t>(tmp, 256)))) return rc;
+    j->rec_tmp = tmp;
+    j->rec_names = names != nullptr;
+    j->rec_ids = read_id != nullptr;
+    j->rec_stride = names ? name_stride : 0;
+    j->rec_prefix_len = (int)plen;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SRM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+long long srm_job_records(srm_job *j, const int64_t *pos, const int32_t *read_id,
+                          const char *names, int name_stride, size_t n_names,
+                          const char *name_prefix, char *out, size_t outcap, long long *n_hits)
+{
+    // this call's inputs replace those of a record stage set up before
+    if (j->rec_stage) {
+        destroy_graph(j);
+        j->rec_stage = false;
+    }
+    int rc = rec_inputs(j, pos, read_id, names, name_stride, n_names, name_prefix);
+    if (rc) return rc;
+    srm_ctx *ctx = j->ctx;
+    const int n = j->n;
+    srm::RecArgs a = rec_args(j, true);
     HIP_TRY(hipMemsetAsync(a.hits, 0, 4, ctx->stream));
     HIP_TRY(srm::launch_rec_measure(a, ctx->stream));
-    size_t tmp = 0;
-    HIP_TRY(srm::rec_scan(nullptr, tmp, a.len, (uint64_t *)j->d_rec_off.p, n, ctx->stream));
-    if ((rc = ctx->ensure(j->d_rec_tmp, std::max<size_t>(tmp, 256)))) return rc;
+    size_t tmp = j->rec_tmp;
     HIP_TRY(srm::rec_scan(j->d_rec_tmp.p, tmp, a.len, (uint64_t *)j->d_rec_off.p, n, ctx->stream));
     uint64_t total = 0;
     uint32_t hits = 0;
@@ -893,9 +1089,52 @@ long long srm_job_records(srm_job *j, const int64_t *pos, const int32_t *read_id
     return (long long)total;
 }
 
+int srm_job_set_records(srm_job *j, const int64_t *pos, const int32_t *read_id, const char *names,
+                        int name_stride, size_t n_names, const char *name_prefix)
+{
+    destroy_graph(j);   // the launch sequence changes
+    j->rec_stage = false;
+    j->rec_done = false;
+    int rc = rec_inputs(j, pos, read_id, names, name_stride, n_names, name_prefix);
+    if (rc) return rc;
+    // the output holds the longest text the lines can have: the name, two int32
+    // coordinates (11 characters each), the strand, both rows of at most w + k
+    // characters, the int16 score and 7 separators per candidate
+    const size_t name_max = names ? (size_t)name_stride - 1 : (size_t)j->rec_prefix_len + 11;
+    j->rec_cap = j->rec_rows_bound + (size_t)j->n * (name_max + 11 + 11 + 2 + 6 + 7);
+    if ((rc = j->ctx->ensure(j->d_rec_out, std::max<size_t>(j->rec_cap, 256)))) return rc;
+    j->rec_stage = true;
+    return SRM_OK;
+}
+
+long long srm_job_records_text(srm_job *j, char *out, size_t outcap, long long *n_hits)
+{
+    if (!j->rec_stage || !j->rec_done)
+        return fail(SRM_ESTATE, "srm_job_records_text without a run of a record stage");
+    srm_ctx *ctx = j->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint64_t total = 0;
+    uint32_t hits = 0;
+    HIP_TRY(hipMemcpyAsync(&total, (const uint64_t *)j->d_rec_off.p + j->n, 8,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&hits, j->d_rec_hits.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (total > j->rec_cap) return fail(SRM_EDEVICE, "record text %llu exceeds its bound %zu",
+                                        (unsigned long long)total, j->rec_cap);
+    if (n_hits) *n_hits = hits;
+    if (out && total <= outcap && total) {
+        HIP_TRY(hipMemcpyAsync(out, j->d_rec_out.p, total, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return (long long)total;
+}
+
 float srm_job_last_ms(srm_job *j) { return j->last_ms; }
 
-float srm_job_kernel_ms(srm_job *j, int kind) { return kind == 0 ? j->band_ms : j->plain_ms; }
+float srm_job_kernel_ms(srm_job *j, int kind)
+{
+    return kind == 0 ? j->band_ms : kind == 1 ? j->plain_ms : j->rec_ms;
+}
 
 int srm_job_use_graph(srm_job *j, int enable)
 {
@@ -1004,17 +1243,57 @@ srm_ctx *dropin_ctx(int which)
         int dev = dropin_device();
         if (dev < 0) return nullptr;
         srm_ctx *c = nullptr;
-        if (srm_ctx_create(dev, cuda_memory_per_thread, &c) != SRM_OK) return nullptr;
+        if (srm_ctx_create(dev, 0, &c) != SRM_OK) return nullptr;
         g_dropin_ctx[which] = c;
     }
     return g_dropin_ctx[which];
 }
 
-// id -> name map built by walking read_names in insertion order (util.h:62-66)
-void collect_names(std::unordered_map<int, const char *> &m)
-{
-    for (read_name_hash *e = read_names; e; e = (read_name_hash *)e->hh.next) m[e->read_number] = e->name;
-}
+// Name of a read id in read_names (util.h:55, 62-66), as the reference finds it per
+// hit (HASH_FIND_INT, cuda-functions.cu:984-985): the key's bucket in the uthash
+// table, then its chain.  The bucket index is uthash's HASH_FNV (the hash SARUMAN
+// is built with, src/Makefile:94, 99): FNV-1 over the key's 4 bytes, each byte
+// taken as a signed char, masked by the power-of-two bucket count.  A table
+// without buckets, or one built with another hash function, misses; then the
+// lookup walks the insertion-order list once and answers the rest of the call
+// from that map.
+class NameLookup {
+  public:
+    const char *find(int id)
+    {
+        if (!read_names) return "";
+        if (!walked) {
+            if (const read_name_hash *e = in_bucket(id)) return e->name;
+            for (read_name_hash *e = read_names; e; e = (read_name_hash *)e->hh.next)
+                map.emplace(e->read_number, e->name);
+            walked = true;
+        }
+        auto it = map.find(id);
+        return it != map.end() ? it->second : "";
+    }
+
+  private:
+    bool walked = false;
+    std::unordered_map<int, const char *> map;
+
+    static const read_name_hash *in_bucket(int id)
+    {
+        const srm_ut_hash_table *tbl = (const srm_ut_hash_table *)read_names->hh.tbl;
+        if (!tbl || !tbl->buckets || tbl->num_buckets == 0 ||
+            (tbl->num_buckets & (tbl->num_buckets - 1)) != 0)
+            return nullptr;
+        unsigned char key[sizeof(int)];
+        std::memcpy(key, &id, sizeof key);
+        uint32_t hv = 2166136261u;
+        for (unsigned char b : key) hv = (hv * 16777619u) ^ (uint32_t)(int32_t)(signed char)b;
+        const srm_ut_hash_handle *hh = tbl->buckets[hv & (tbl->num_buckets - 1)].hh_head;
+        for (; hh; hh = (const srm_ut_hash_handle *)hh->hh_next) {
+            if (hh->keylen == sizeof(int) && std::memcmp(hh->key, key, sizeof(int)) == 0)
+                return (const read_name_hash *)((const char *)hh - tbl->hho);
+        }
+        return nullptr;
+    }
+};
 }  // namespace
 
 extern "C" {
@@ -1119,6 +1398,13 @@ void cuda_needleman_wunsch(int match, int mismatch, char *genome_sequences_on_ho
         fprintf(stderr, "ERROR: error bound %d outside 0..255\n", k);
         return;
     }
+    // the genome array holds every window plus k slot bytes per candidate
+    // (hash.c:1095, cuda-functions.cu:734)
+    if (total_genome_array_size < (size_t)sample_size * (size_t)error) {
+        fprintf(stderr, "ERROR: genome array of %zu bytes is smaller than its %d slots of %d\n",
+                total_genome_array_size, sample_size, error);
+        return;
+    }
     const size_t win_total = total_genome_array_size - (size_t)sample_size * (size_t)error;
     const size_t read_total = total_read_sequence_array_size;
 
@@ -1166,14 +1452,24 @@ void cuda_needleman_wunsch(int match, int mismatch, char *genome_sequences_on_ho
     srm_pack_codes(read_sequences_on_host, read_total, rw.data(), SRM_CODE_PAD);
     srm_pack_codes(genome_sequences_on_host, win_total, ww.data(), SRM_CODE_PAD);
 
+    // the pool the reference sized per strand (saruman.c:82) bounds a batch; the
+    // reference never compared the two (cuda-functions.cu:648-665, Q19)
+    const char *kenv0 = getenv("SARUMAN_KERNEL");
+    const int kernel0 = kenv0 && !strcmp(kenv0, "plain") ? SRM_KERNEL_PLAIN : SRM_KERNEL_AUTO;
+    const size_t need = srm_job_bytes_bound(ctx, match, mismatch, cand.data(), n, rw.size(), ww.size(),
+                                            any_bytes ? read_total : 0, any_bytes ? win_total : 0,
+                                            kernel0, 0, 0, 0);
+    if (cuda_memory_per_thread && need > cuda_memory_per_thread) {
+        fprintf(stderr, "ERROR: batch of %d candidates needs %zu bytes, the pool holds %lu\n", n,
+                need, cuda_memory_per_thread);
+        return;
+    }
     if (!ctx->dropin_job && srm_job_create(ctx, &ctx->dropin_job) != SRM_OK) {
         fprintf(stderr, "ERROR: %s\n", srm_last_error());
         return;
     }
     srm_job *job = ctx->dropin_job;
-    const char *kenv = getenv("SARUMAN_KERNEL");
-    int kernel = SRM_KERNEL_AUTO;
-    if (kenv && !strcmp(kenv, "plain")) kernel = SRM_KERNEL_PLAIN;
+    const int kernel = kernel0;
     int rc = srm_job_upload(job, match, mismatch, cand.data(), n, rw.data(), rw.size(), ww.data(),
                             ww.size(), any_bytes ? read_sequences_on_host : nullptr,
                             any_bytes ? read_total : 0, any_bytes ? genome_sequences_on_host : nullptr,
@@ -1189,15 +1485,13 @@ void cuda_needleman_wunsch(int match, int mismatch, char *genome_sequences_on_ho
     alignment_time[which] += srm_job_last_ms(job);
 
     // host post-processing and records, cuda-functions.cu:828-1006
-    std::unordered_map<int, const char *> names;
-    collect_names(names);
+    NameLookup names;
     std::string out;
     std::vector<char> line(4096);
     for (int t = 0; t < n; ++t) {
         if (res[t].score > k) continue;
         const srm_cand &c = cand[t];
-        auto it = names.find(read_id_array[t]);
-        const char *nm = it != names.end() ? it->second : "";
+        const char *nm = names.find(read_id_array[t]);
         int len = srm_format_record(nm, (unsigned)genome_pos_array[t], c.flag, k, c.L, c.w, &res[t],
                                     ops.data() + c.ops_off, read_sequences_on_host + rstart[t],
                                     genome_sequences_on_host + gstart[t], line.data(), (int)line.size());

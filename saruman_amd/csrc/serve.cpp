@@ -6,6 +6,8 @@
 //                            (on the host, or on the device with its own stream)
 //   admit    (host thread)   a request enters as soon as one of n_slots slots is
 //                            free and its device footprint fits the HBM budget
+//                            (below: what the slots hold counts, not only what
+//                            requests are charged)
 //   slot k   (host thread)   upload, persistent banded kernel, download and record
 //                            formatting on the slot's own HIP stream
 // so the filtering of later requests, the device work of several requests and
@@ -74,6 +76,13 @@ struct Request {
     bool done = false;
 };
 
+// a request's failure: its status and the library's message for it
+void set_failed(srm_request_stats &st, int rc, const char *msg)
+{
+    st.status = rc;
+    snprintf(st.message, sizeof st.message, "%s", msg ? msg : "");
+}
+
 }  // namespace
 
 struct srm_server {
@@ -92,7 +101,16 @@ struct srm_server {
     std::map<uint64_t, std::shared_ptr<Request>> all;
     std::vector<std::shared_ptr<Request>> slot_req;
     std::vector<bool> slot_busy;
-    size_t budget_used = 0;
+    // HBM accounting.  A slot's context holds its job's buffers between requests
+    // (they only grow), and slot 0 also holds the resident genome (base[0]).  An
+    // idle slot counts with what it holds; a busy one with its reservation, the
+    // bytes it held at admission plus the request's footprint, which its context's
+    // pool cap enforces (an allocation past it fails).  Admission keeps the sum
+    // within hbm_budget, releasing idle slots' buffers when that makes room.
+    std::vector<size_t> base;         // bytes a slot holds with its job released
+    std::vector<size_t> reserved;     // busy slots: the pool cap granted
+    size_t retain = 0;                // an idle slot keeps at most base + this
+    size_t peak_accounted = 0;        // largest sum seen at admission or completion
     uint64_t next_id = 1;
     bool stop = false;
     bool hold = false;                // admission paused (srm_server_hold)
@@ -102,6 +120,14 @@ struct srm_server {
 
     void filter_loop();
     void admit_loop();
+    size_t accounted(int s, size_t s_bytes) const;   // under mu
+    int pick_slot(size_t need);                       // under mu; trims idle slots
+    size_t sum_base() const
+    {
+        size_t t = 0;
+        for (size_t b : base) t += b;
+        return t;
+    }
     void slot_loop(int s);
     void prepare(Request &r);
     void run_on_slot(int s, Request &r);
@@ -124,7 +150,7 @@ void srm_server::prepare(Request &r)
         int rc = srm_dfilter_run2(dfilter, nullptr, r.reads.data(), r.off.data(), r.len.data(),
                                   r.k.data(), n, cfg.chunk_value, &both[0], &both[1]);
         if (rc != SRM_OK) {
-            r.st.status = rc;
+            set_failed(r.st, rc, srm_last_error());
             return;
         }
     }
@@ -136,7 +162,7 @@ void srm_server::prepare(Request &r)
                                     cfg.chunk_value, cfg.filter_threads, &F);
             if (rc != SRM_OK) {
                 if (both[1]) srm_filter_destroy(both[1]);
-                r.st.status = rc;
+                set_failed(r.st, rc, srm_last_error());
                 return;
             }
         }
@@ -174,18 +200,14 @@ void srm_server::prepare(Request &r)
     r.read_words.assign((2 * total + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
     srm_pack_codes(r.reads.data(), total, r.read_words.data(), SRM_CODE_PAD);
     srm_pack_codes(r.reads_rev.data(), total, r.read_words.data(), SRM_CODE_PAD + total);
-    // device footprint: inputs, results, transcripts, tiles and the band scratch
-    // every tile keeps until its traceback (at most 64 diagonals per lane; a tile
-    // per 64 candidates plus one partial tile per band class and L mod 8)
     size_t maxL = 0;
     for (auto L : r.len) maxL = std::max<size_t>(maxL, L);
     r.max_len = maxL;
     const size_t nc = r.cand.size();
-    const size_t tiles = (nc + 63) / 64 + 64;
-    r.dev_bytes = r.read_words.size() * 4 + nc * (sizeof(srm_cand) + sizeof(srm_result) + 4 * 4) +
-                  r.ops_total * 4 + tiles * 64 * ((maxL + 8) / 8) * 64 * 4 + (1u << 20);
-    if (cfg.gpu_records)   // positions, ids, lengths, offsets, entries and the text
-        r.dev_bytes += nc * (8 + 4 + 8 + 8 + 16 + 2 * maxL + 48);
+    // device footprint: the library's bound for this job, record stage included
+    // when the records are formatted on the device (names "r" + decimal id)
+    r.dev_bytes = srm_job_bytes_bound(nullptr, 0, 1, r.cand.data(), (int)nc, r.read_words.size(), 0,
+                                      0, 0, cfg.kernel, cfg.gpu_records, 2, 1 + 11);
     r.st.n_reads = (long long)n;
     r.st.n_candidates = (long long)nc;
     r.st.dev_bytes = (long long)r.dev_bytes;
@@ -210,7 +232,7 @@ void srm_server::run_on_slot(int s, Request &r)
     }
     r.st.t_aligned = now();
     if (rc != SRM_OK) {
-        r.st.status = rc;
+        set_failed(r.st, rc, srm_last_error());
         return;
     }
     // records: perfect hits (hash.c:580) of each strand, then aligned hits in
@@ -270,7 +292,7 @@ void srm_server::run_on_slot(int s, Request &r)
                                     r.rec_text.get(), (size_t)total, &hits);
         }
         if (total < 0) {
-            r.st.status = (int)total;
+            set_failed(r.st, (int)total, srm_last_error());
             return;
         }
         r.rec_len = (size_t)total;
@@ -284,7 +306,7 @@ void srm_server::run_on_slot(int s, Request &r)
                                             r.roff_bytes.data(), r.rid.data(), "r", nullptr, 0,
                                             &hits, cfg.format_threads);
         if (need < 0) {
-            r.st.status = SRM_EINVAL;
+            set_failed(r.st, SRM_EINVAL, "host record formatting failed");
             return;
         }
         size_t base = out.size();
@@ -320,8 +342,9 @@ void srm_server::filter_loop()
             if (r->st.status != SRM_OK) {
                 r->done = true;
                 r->st.t_done = now();
-            } else if (r->dev_bytes > cfg.hbm_budget) {
-                r->st.status = SRM_ENOMEM;   // can never be admitted
+            } else if (sum_base() + r->dev_bytes > cfg.hbm_budget) {
+                // cannot fit even with every slot released
+                set_failed(r->st, SRM_ENOMEM, "request exceeds the HBM budget");
                 r->done = true;
                 r->st.t_done = now();
             } else {
@@ -332,6 +355,46 @@ void srm_server::filter_loop()
     }
 }
 
+// the HBM the slots account for, with slot s (if >= 0) counted as s_bytes: idle
+// slots with what their context holds (nothing else touches an idle slot's
+// context), busy slots with their reservation
+size_t srm_server::accounted(int s, size_t s_bytes) const
+{
+    size_t t = 0;
+    for (size_t i = 0; i < ctx.size(); ++i) {
+        if ((int)i == s) t += s_bytes;
+        else if (slot_busy[i]) t += reserved[i];
+        else t += srm_ctx_pool_used(ctx[i]);
+    }
+    return t;
+}
+
+// an idle slot for a request of `need` bytes, or -1.  Prefers the slot holding the
+// most (its buffers are likely reused); when nothing fits, releases idle slots'
+// job buffers, largest first, if that makes room.
+int srm_server::pick_slot(size_t need)
+{
+    std::vector<int> idle;
+    for (size_t i = 0; i < ctx.size(); ++i)
+        if (!slot_busy[i]) idle.push_back((int)i);
+    if (idle.empty()) return -1;
+    std::sort(idle.begin(), idle.end(), [&](int a, int b) {
+        return srm_ctx_pool_used(ctx[(size_t)a]) > srm_ctx_pool_used(ctx[(size_t)b]);
+    });
+    for (int s : idle)
+        if (accounted(s, srm_ctx_pool_used(ctx[(size_t)s]) + need) <= cfg.hbm_budget) return s;
+    // with every idle slot released, does it fit on the first?
+    const int s = idle[0];
+    size_t floor = 0;
+    for (size_t i = 0; i < ctx.size(); ++i) floor += slot_busy[i] ? reserved[i] : base[i];
+    if (floor + need > cfg.hbm_budget) return -1;
+    for (int i : idle) {
+        if (srm_ctx_pool_used(ctx[(size_t)i]) > base[(size_t)i]) srm_job_release(job[(size_t)i]);
+        if (accounted(s, srm_ctx_pool_used(ctx[(size_t)s]) + need) <= cfg.hbm_budget) return s;
+    }
+    return s;   // unreachable: all idle slots are at their base now
+}
+
 void srm_server::admit_loop()
 {
     for (;;) {
@@ -340,20 +403,18 @@ void srm_server::admit_loop()
         cv.wait(lk, [&] {
             if (stop) return true;
             if (hold || ready.empty()) return false;
-            if (budget_used + ready.front()->dev_bytes > cfg.hbm_budget) return false;
-            for (size_t s = 0; s < slot_busy.size(); ++s)
-                if (!slot_busy[s]) {
-                    slot = (int)s;
-                    return true;
-                }
-            return false;
+            slot = pick_slot(ready.front()->dev_bytes);
+            return slot >= 0;
         });
         if (stop) return;
         auto r = ready.front();
         ready.pop_front();
-        budget_used += r->dev_bytes;
-        slot_busy[(size_t)slot] = true;
-        slot_req[(size_t)slot] = r;
+        const size_t s = (size_t)slot;
+        reserved[s] = srm_ctx_pool_used(ctx[s]) + r->dev_bytes;
+        srm_ctx_set_pool_cap(ctx[s], reserved[s]);
+        slot_busy[s] = true;
+        peak_accounted = std::max(peak_accounted, accounted(-1, 0));
+        slot_req[s] = r;
         r->st.slot = slot;
         r->st.t_admitted = now();
         lk.unlock();
@@ -372,13 +433,16 @@ void srm_server::slot_loop(int s)
             r = slot_req[(size_t)s];
         }
         run_on_slot(s, *r);
+        // an idle slot keeps its buffers for the next request, up to `retain`
+        if (srm_ctx_pool_used(ctx[(size_t)s]) > base[(size_t)s] + retain) srm_job_release(job[(size_t)s]);
         {
             std::lock_guard<std::mutex> lk(mu);
             r->st.t_done = now();
             r->done = true;
-            budget_used -= r->dev_bytes;
             slot_req[(size_t)s] = nullptr;
             slot_busy[(size_t)s] = false;
+            reserved[(size_t)s] = 0;
+            peak_accounted = std::max(peak_accounted, accounted(-1, 0));
             // inputs are no longer needed
             std::vector<srm_cand>().swap(r->cand);
             std::vector<uint32_t>().swap(r->read_words);
@@ -441,6 +505,13 @@ int srm_server_create(const srm_server_config *cfg, const char *genome, size_t g
     }
     S->slot_req.assign((size_t)cfg->n_slots, nullptr);
     S->slot_busy.assign((size_t)cfg->n_slots, false);
+    S->reserved.assign((size_t)cfg->n_slots, 0);
+    for (srm_ctx *c : S->ctx) S->base.push_back(srm_ctx_pool_used(c));
+    S->retain = S->cfg.hbm_budget / (2 * (size_t)cfg->n_slots);
+    if (S->sum_base() > S->cfg.hbm_budget) {
+        srm_server_destroy(S);
+        return SRM_ENOMEM;   // the resident genome alone exceeds the budget
+    }
     S->threads.emplace_back([S] { S->filter_loop(); });
     S->threads.emplace_back([S] { S->admit_loop(); });
     for (int s = 0; s < cfg->n_slots; ++s) S->threads.emplace_back([S, s] { S->slot_loop(s); });
@@ -511,6 +582,15 @@ long long srm_server_result(srm_server *S, long long id, char *out, size_t outca
     }
     if (release && out && (size_t)need <= outcap) S->all.erase(it);
     return need;
+}
+
+/* HBM the alignment slots account for now (held by idle slots, reserved by busy
+ * ones) and the most they have accounted for at once; both stay within hbm_budget */
+void srm_server_memory(srm_server *S, size_t *now_bytes, size_t *peak_bytes)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (now_bytes) *now_bytes = S->accounted(-1, 0);
+    if (peak_bytes) *peak_bytes = S->peak_accounted;
 }
 
 /* on != 0 pauses admission: filtered requests wait in the ready queue and

@@ -29,10 +29,13 @@ class RequestStats(C.Structure):
                 ("device_ms", C.c_double), ("n_reads", C.c_longlong),
                 ("n_candidates", C.c_longlong), ("n_hits", C.c_longlong),
                 ("n_perfect", C.c_longlong), ("n_mapped", C.c_longlong),
-                ("dev_bytes", C.c_longlong), ("slot", C.c_int), ("status", C.c_int)]
+                ("dev_bytes", C.c_longlong), ("slot", C.c_int), ("status", C.c_int),
+                ("message", C.c_char * 160)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["message"] = self.message.decode(errors="replace")
+        return d
 
 
 def qs_for_lengths(Lmin: int, Lmax: int, k=None, k_rate=0.05):
@@ -78,12 +81,19 @@ class Server:
         buf = C.create_string_buffer(max(int(need), 1))
         self.lib.srm_server_result(self.h, rid, buf, len(buf), C.byref(st), int(release))
         if st.status != 0:
-            raise RuntimeError(f"request {rid} failed with status {st.status}")
+            raise RuntimeError(f"request {rid} failed with status {st.status}: "
+                               f"{st.message.decode(errors='replace')}")
         return buf.raw[:need].decode(), st.as_dict()
 
     def hold(self, on: bool = True) -> None:
         """Pauses (True) or resumes (False) admission; running requests finish."""
         self.lib.srm_server_hold(self.h, int(on))
+
+    def memory(self):
+        """(HBM the alignment slots account for now, the most at once), bytes"""
+        a, b = C.c_size_t(), C.c_size_t()
+        self.lib.srm_server_memory(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def queue(self):
         """(not yet filtered, waiting for admission, on a slot)"""

@@ -388,6 +388,38 @@ def device_batch(reads: Reads, cands_list, read_words=None) -> DeviceBatch:
     return DeviceBatch(cand, read_words, ops_total, rid, pos, flag, k, L, w)
 
 
+def subset_reads(reads: Reads, lo: int, hi: int) -> Reads:
+    """Reads lo..hi-1 with their bases copied out and offsets from 0."""
+    a = np.frombuffer(reads.seq, np.uint8) if not isinstance(reads.seq, np.ndarray) else reads.seq
+    b0 = int(reads.off[lo]) if hi > lo else 0
+    b1 = int(reads.off[hi - 1] + reads.length[hi - 1]) if hi > lo else 0
+    return Reads(a[b0:b1].tobytes(), (reads.off[lo:hi] - b0).astype(np.uint64),
+                 reads.length[lo:hi], reads.true_pos[lo:hi], reads.strand[lo:hi],
+                 reads.k[lo:hi])
+
+
+def batches_by_reads(reads: Reads, strands, batch_reads: int):
+    """The candidates of both strands cut by read id into batches of batch_reads
+    reads.  Returns [(lo, hi, DeviceBatch)]: the batch's read ids are local (read
+    lo is 0), its read codes hold reads lo..hi-1 and their reverse complements."""
+    out = []
+    for lo in range(0, reads.n, batch_reads):
+        hi = min(reads.n, lo + batch_reads)
+        sub = subset_reads(reads, lo, hi)
+        parts = []
+        for s in strands:
+            m = (s.read >= lo) & (s.read < hi)
+            pm = (s.perfect_read >= lo) & (s.perfect_read < hi)
+            parts.append(StrandCands(s.do_reverse, (s.read[m] - lo).astype(np.int32), s.pos[m],
+                                     s.flag[m], s.w[m], np.zeros(0, np.uint64),
+                                     (s.perfect_read[pm] - lo).astype(np.int32),
+                                     s.perfect_pos[pm]))
+        db = device_batch(sub, parts)
+        db.extra["perfect_read"] = np.unique(np.concatenate([p.perfect_read for p in parts]))
+        out.append((lo, hi, db))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # the benchmark configurations
 # ---------------------------------------------------------------------------

@@ -90,3 +90,45 @@ def test_device_records_prefix_names_and_no_hits():
         assert hits2 == n_hits and (n_hits > 0 or text2 == b"")
     finally:
         job.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_record_stage_in_the_launch_sequence(orc, graph):
+    """srm_job_set_records makes the record writer a stage of every run: the text
+    of each run (eager, or replayed from a hipGraph several times with eager work
+    and synchronisations between the replays) equals the standalone writer's and
+    the oracle's records."""
+    print(f"[gpu] record stage graph={graph}", flush=True)
+    from saruman_amd.batch import Job, _layout, context, name_table, pack_codes
+    rng = np.random.default_rng(24)
+    batch = random_batch(rng, 2500, L=(50, 250), k=13, per_cand_k=lambda L: math.ceil(0.05 * L))
+    lay = _layout(batch)
+    want = orc.align_batch(batch)
+    want_text = "".join(r for r in want.records if r).encode()
+    names = name_table(batch.names)
+    rid = np.arange(batch.n, dtype=np.int32)
+    job = Job(context(0))
+    other = Job(context(0))
+    try:
+        job.upload(0, 1, lay["cand"], pack_codes(lay["rcat"]), pack_codes(lay["wcat"]),
+                   ops_total=lay["ops_total"])
+        job.set_records(batch.pos, read_id=rid, names=names)
+        job.use_graph(graph)
+        for rep in range(3):
+            job.run("banded")
+            text, hits = job.records_text()
+            assert text == want_text, rep
+            assert hits == sum(1 for r in want.records if r)
+            # eager work on the same device between replays
+            small = random_batch(rng, 300, L=36, k=2)
+            sl = _layout(small)
+            other.upload(0, 1, sl["cand"], pack_codes(sl["rcat"]), pack_codes(sl["wcat"]),
+                         ops_total=sl["ops_total"])
+            other.run("auto")
+            other.download()
+        # the standalone writer on the same run gives the same text and removes the stage
+        text2, _ = job.records(batch.pos, read_id=rid, names=names)
+        assert text2 == want_text
+    finally:
+        job.close()
+        other.close()

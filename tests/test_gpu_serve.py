@@ -130,7 +130,8 @@ def test_over_budget_request_fails_cleanly(data):
     print("[gpu] serve over budget", flush=True)
     genome, pool = data
     qs = serve.qs_for_lengths(36, 250)
-    with serve.Server(genome, qs, n_slots=2, chunk_value=5000, hbm_budget=48 << 20) as S:
+    # small requests need about 5.5 MiB, the 30,000-read one about 37.5 MiB
+    with serve.Server(genome, qs, n_slots=2, chunk_value=5000, hbm_budget=24 << 20) as S:
         small = S.submit(pool, 0, 200)
         big = S.submit(pool, 1000, 31000)
         small2 = S.submit(pool, 200, 400)
