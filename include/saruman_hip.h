@@ -332,6 +332,31 @@ int srm_format_record(const char *name, unsigned int pos, int flag, int k, int L
 int srm_slot_rows(const srm_result *res, const uint32_t *ops, const char *read, const char *win,
                   char *R, char *G, int cap);
 
+/*
+ * srm_job_upload + srm_job_run(SRM_KERNEL_BANDED) + the results, allocating
+ * nothing.  No counterpart in the reference: this is for callers that cannot
+ * allocate -- code inside a hipGraph capture, tight serving loops, harnesses that
+ * forbid allocation.  The caller passes the candidate descriptors on the host
+ * (srm_cand, as for srm_job_upload) and device pointers to the read and window
+ * codes (SRM_CODE_PAD codes of padding at both ends, as srm_job_upload takes
+ * them), the results (n srm_result) and the transcripts (n_ops_words words, each
+ * candidate's slot at its ops_off).  Every candidate must fit the exact band
+ * (match >= 0, mismatch >= 1, k <= SRM_BAND_KMAX, band <= SRM_BAND_MAX, w >= L - k,
+ * 2-bit codes): the plain kernel's matrices are not provided for.  Everything else
+ * the kernels use lives in one static device array of SRM_STATIC_SCRATCH_BYTES per
+ * GPU, held until the process ends; at most SRM_STATIC_MAX_CAND candidates, and a
+ * batch whose band scratch does not fit is refused with SRM_ENOMEM (split it).
+ * Calls are serialised and synchronous on `stream` (a hipStream_t, NULL for the
+ * default stream): they never overlap, and each finds the scratch as the previous
+ * call left it.
+ */
+#define SRM_STATIC_SCRATCH_BYTES ((size_t)128 << 20)
+#define SRM_STATIC_MAX_CAND 65536
+int srm_align_static(int device, void *stream, int match, int mismatch, const srm_cand *cand,
+                     int n, const uint32_t *d_read_words, size_t n_read_words,
+                     const uint32_t *d_win_words, size_t n_win_words, srm_result *d_res,
+                     uint32_t *d_ops, size_t n_ops_words);
+
 /* page-locked host memory (fast transfers to and from the device); NULL on failure */
 void *srm_host_alloc(size_t bytes);
 void srm_host_free(void *p);

@@ -22,7 +22,7 @@ LIB = LIBDIR / "libsaruman_hip.so"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["nw_kernels.hip", "band_bits.hip", "nw_runtime.hip", "records.cpp", "filter.cpp", "filter_gpu.hip",
-               "records_gpu.hip", "synth.cpp", "serve.cpp", "ingest.cpp"]
+               "records_gpu.hip", "synth.cpp", "serve.cpp", "ingest.cpp", "static_align.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

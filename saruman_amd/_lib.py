@@ -58,6 +58,8 @@ _PROTOS = [
     ("srm_job_path_count", C.c_int, [_vp, C.c_int]),
     ("srm_job_records", C.c_longlong, [_vp, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_char_p, _vp,
                                        C.c_size_t, _vp]),
+    ("srm_align_static", C.c_int, [C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_size_t,
+                                   _vp, C.c_size_t, _vp, _vp, C.c_size_t]),
     ("srm_job_set_records", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_size_t, C.c_char_p]),
     ("srm_job_records_text", C.c_longlong, [_vp, _vp, C.c_size_t, _vp]),
     ("srm_format_record", C.c_int, [C.c_char_p, C.c_uint, C.c_int, C.c_int, C.c_int, C.c_int,

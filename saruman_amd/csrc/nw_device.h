@@ -72,6 +72,23 @@ struct RecArgs {
     char *out;
 };
 
+// band height for the banded kernels: d = row - col in [-k, max(2s+k, w-L)]
+inline int band_cells(const srm_cand &c)
+{
+    const int k = c.k;
+    const int s = (c.flag <= 7 && (c.flag & 2)) ? k : 0;
+    const int hi = 2 * s + k > (int)c.w - (int)c.L ? 2 * s + k : (int)c.w - (int)c.L;
+    return hi + k + 1;
+}
+
+// words of one tile's nibble band scratch (other costs): 8*NW diagonals x 8-column
+// blocks x 64 lanes, then the tail (columns L - 1 and L packed by diagonal,
+// equal-character bits)
+inline size_t tile_scratch_words(int nw, int maxL)
+{
+    return ((size_t)nw * 8 * ((maxL + 8) / 8) + 2 * (size_t)nw + 2) * 64;
+}
+
 hipError_t launch_plain(const PlainArgs &a, bool codes, hipStream_t st);
 hipError_t launch_rec_measure(const RecArgs &a, hipStream_t st);
 hipError_t launch_rec_write(const RecArgs &a, hipStream_t st);

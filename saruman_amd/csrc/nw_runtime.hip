@@ -193,21 +193,8 @@ struct srm_job {
 
 namespace {
 
-// band height for the banded kernel: d = row - col in [-k, max(2s+k, w-L)]
-inline int band_cells(const srm_cand &c)
-{
-    int k = c.k;
-    int s = (c.flag <= 7 && (c.flag & 2)) ? k : 0;
-    int hi = std::max(2 * s + k, (int)c.w - (int)c.L);
-    return hi + k + 1;
-}
-
-// words of one tile's band scratch: 8*NW diagonals x 8-column blocks x 64 lanes,
-// then the tail (columns L - 1 and L packed by diagonal, equal-character bits)
-inline size_t tile_scratch_words(int nw, int maxL)
-{
-    return ((size_t)nw * 8 * ((maxL + 8) / 8) + 2 * (size_t)nw + 2) * 64;
-}
+using srm::band_cells;
+using srm::tile_scratch_words;
 
 void destroy_graph(srm_job *j)
 {
