@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
-"""Benchmark: reads mapped/s of the batched persistent banded kernel vs the plain
+"""Benchmark: reads mapped/s of the batched banded kernels against the plain
 one-alignment-per-thread kernel, on one MI355X.
 
-A step = one pass of the alignment hot path (DP fill + traceback of every
-candidate of the workload, results left in HBM) over the whole candidate set of
-the workload, with the packed genome resident and the candidate batch already
-uploaded.  Default workload: the mixed-length one (4 Mbp genome with 2% repeat
-copies, 2,000,000 reads of 50-250 bp, k = ceil(5% L), both strands).
+A step is one pass of the alignment hot path over one batch of candidates, from
+candidate arrays resident in HBM to the TSV records of every hit in HBM: the DP
+fill and traceback of every candidate (the band kernels) and the reference's
+post-processing and record formatting (the record kernels), one hipGraph replay
+per step.  The packed genome, the candidates and the read codes are uploaded
+before the timed region.  The PCIe-inclusive rate (batches uploaded from and
+records copied back to page-locked host memory, transfers overlapped with the
+device work of other batches) is reported beside it as `host_to_host`.
 
-Prints one JSON line (rank 0).  `value` = reads mapped per second of the batched
-kernel: distinct reads with at least one record (perfect hit from the filter or
-aligned hit with score <= k) divided by the step time.
+Default workload: the mixed-length one (4 Mbp genome with 2% repeat copies,
+2,000,000 reads of 50-250 bp, k = ceil(5% L), both strands), one batch.
+`--workload scale`: 12 Mbp two-contig genome, 10,000,000 reads of 150 bp, k 8,
+in batches of 1,000,000 reads (one job and one graph per batch; the steps cycle
+over the batches).
+
+Prints one JSON line.  `value` = reads mapped per second: distinct reads with at
+least one record (a perfect hit from the filter or an aligned hit with score <= k)
+over the step time.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -25,6 +35,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "reads mapped/sec + GCUPS, batched vs plain kernel; p50/p99 request latency"
 PEAK_INT32_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz, int32 VALU ops/s
 PEAK_HBM_GBS = 8000.0
 OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
@@ -32,7 +43,7 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 # HBM bytes of one step of the band phase (bits_band_kernel and bits_trace_kernel of every
 # class), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.000 and x1.000
 # by a 256 MiB copy kernel), for the exact configuration profiled: workload,
-# candidates, launch tuning and the sha256 prefix of the kernel source.
+# candidates, launch tuning and the sha256 prefix of the kernel sources.
 # Source: profiles/bits_band_and_trace_kernels_mixed_2M_reads.md
 PMC_TRAFFIC = [
     {"workload": "mixed", "candidates": 1774238, "tuning": "defaults",
@@ -68,6 +79,7 @@ def profiled_traffic(workload, candidates, tuning):
             return e["bytes"]
     return None
 
+
 WORKLOAD_TEXT = {
     "small": "1 Mbp uniform genome, 100,000 x 36 bp reads, 0-2 errors, k=2, both strands",
     "single": "5 Mbp 50% GC genome, 1,000,000 x 100 bp reads (sub 1%, indel 0.1%, 5% random), "
@@ -75,8 +87,9 @@ WORKLOAD_TEXT = {
     "mixed": "4 Mbp genome with 200 repeat copies (2%, 0.5% divergence), 2,000,000 reads "
              "50-250 bp (sub 1%, indel 0.1%), k=ceil(5% L), both strands",
     "scale": "12 Mbp genome in 2 contigs (8+4 Mbp, 45% GC), 10,000,000 x 150 bp reads "
-             "(sub 1%, indel 0.2%), k=8, both strands",
+             "(sub 1%, indel 0.2%), k=8, both strands, batches of 1,000,000 reads",
 }
+BATCH_READS = {"scale": 1_000_000}
 
 
 def log(*a):
@@ -91,122 +104,115 @@ def band_cells(L, k, flag):
 
 
 def hbm_bytes(L, w, k):
-    """Algorithmic HBM bytes per candidate: 2-bit read and window, 16 B descriptor,
-    2-bit transcript, 8 B result."""
+    """Algorithmic HBM bytes per candidate of the band phase: 2-bit read and window,
+    16 B descriptor, 2-bit transcript, 8 B result."""
     return (np.ceil(L / 4) + np.ceil(w / 4) + 16 + np.ceil(2 * (L + 3 * k) / 8) + 8).astype(
         np.float64)
 
 
-def cpu_baseline(genome, reads, db, n_sample, mapped_per_cand):
-    """The C oracle (full int16 matrix, scalar) over the first n_sample candidates,
-    OpenMP over all host cores."""
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model or platform.processor(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(genome, reads, lo, db, n_sample, runs, mapped_per_cand):
+    """The C oracle (full int16 matrix, scalar per candidate, OpenMP over the host
+    cores) over the first n_sample candidates of the first batch: one warm-up run,
+    then the median of `runs` runs.  Its DP fill and traceback are timed (records
+    are formatted outside the timed call)."""
     from oracle import oracle as orc
     from saruman_amd.batch import Batch
-    from saruman_amd.workload import revcomp_all
+    from saruman_amd.workload import revcomp_all, subset_reads
     n = min(n_sample, db.n)
+    sub = subset_reads(reads, lo, lo + int(db.read_id.max()) + 1)
     g = genome.seq
-    fwd = reads.seq
-    rev = revcomp_all(reads).tobytes()
+    fwd = sub.seq
+    rev = revcomp_all(sub).tobytes()
     wins, rds = [], []
     for t in range(n):
         p, w = int(db.pos[t]), int(db.w[t])
         r = int(db.read_id[t])
-        o, L = int(reads.off[r]), int(reads.length[r])
+        o, L = int(sub.off[r]), int(sub.length[r])
         wins.append(g[p:p + w])
         rds.append((rev if db.flag[t] > 3 else fwd)[o:o + L])
     b = Batch(0, 1, int(db.k[:n].max()), wins, rds, db.flag[:n], db.pos[:n],
               [f"r{int(x)}" for x in db.read_id[:n]], ks=db.k[:n])
-    res = orc.align_batch(b, with_records=False)
-    cand_rate = n / res.seconds
+    orc.align_batch(b, with_records=False)                # warm-up
+    secs = []
+    for _ in range(runs):
+        res = orc.align_batch(b, with_records=False)
+        secs.append(res.seconds)
+    med = float(np.median(secs))
+    cand_rate = n / med
     return dict(value=cand_rate * mapped_per_cand, unit="reads/s", cores=res.threads,
                 kind="port",
-                sample=f"oracle (full int16 matrix) over the first {n} candidates of the same "
-                       f"workload in {res.seconds:.2f} s on {res.threads} threads; "
-                       f"{cand_rate:.0f} candidates/s scaled by the workload's reads mapped "
-                       f"per candidate ({mapped_per_cand:.3f})",
-                candidates_per_s=cand_rate, seconds=res.seconds), res
+                sample=f"oracle (full int16 matrix, DP fill and traceback) over the first {n} "
+                       f"candidates of the workload on {res.threads} OpenMP threads: median of "
+                       f"{runs} runs after one warm-up, {med:.3f} s ({cand_rate:.0f} candidates/s), "
+                       f"scaled by the workload's reads mapped per candidate "
+                       f"({mapped_per_cand:.3f})",
+                candidates_per_s=cand_rate, seconds_runs=secs, host=host_info())
 
 
 def serve_leg(args):
     """Continuous batching over the resident 5 Mbp genome: Poisson arrivals, request
-    sizes log-uniform in 1,000-200,000 reads of 36-250 bp, k = ceil(5% L).  The
-    filter and the record formatting run on the device (the host at the first
-    rate as well, for comparison); arrival rates are swept."""
+    sizes log-uniform in 1,000-200,000 reads of 36-250 bp, k = ceil(5% L); the
+    filter and the record formatting on the device; arrival rates swept."""
     from saruman_amd import serve
     from saruman_amd import workload as wl
     cfg, genome, pool = wl.make_workload("serve", n_reads=400_000)
     rng = np.random.default_rng(cfg["seed"])
-    sizes = serve.request_sizes(rng, args.serve_requests)
     qs = serve.qs_for_lengths(36, 250)
     rates = [float(x) for x in args.serve_rates.split(",")]
 
-    def one(rate, device):
+    def one(rate, device, n_req):
+        sizes = serve.request_sizes(rng, n_req)
         with serve.Server(genome, qs, n_slots=args.serve_slots, chunk_value=cfg["chunk"],
                           gpu_filter=device, gpu_records=device) as S:
             serve.poisson_load(S, pool, 50.0, [2000, 2000], seed=1)     # warm the slots
             res = serve.poisson_load(S, pool, rate, sizes, seed=cfg["seed"])
+            mem_now, mem_peak = S.memory()
         st = res.stats
         dev = sorted((s["t_admitted"], s["t_aligned"]) for s in st)
         overlapped = sum(1 for j in range(1, len(dev)) if dev[j][0] < dev[j - 1][1])
-        out = {"rate_hz": rate, "filter_and_records": "device" if device else "host",
+        out = {"rate_hz": rate, "requests": len(st),
+               "filter_and_records": "device" if device else "host",
                "p50_ms": res.p50 * 1e3, "p99_ms": res.p99 * 1e3,
                "reads_per_s": res.reads / res.wall, "mapped_reads_per_s": res.mapped / res.wall,
                "reads": res.reads, "wall_s": res.wall,
                "device_intervals_overlapping": overlapped,
                "mean_filter_ms": float(np.mean([(s["t_filtered"] - s["t_submit"]) for s in st])) * 1e3,
-               "mean_device_ms": float(np.mean([s["device_ms"] for s in st]))}
+               "mean_device_ms": float(np.mean([s["device_ms"] for s in st])),
+               "hbm_peak_bytes": mem_peak}
         log(f"[bench] serving ({out['filter_and_records']} filter/records) at {rate:g}/s: "
             f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
             f"{out['reads_per_s']:.0f} reads/s over {len(st)} requests")
         return out
 
-    runs = [one(r, True) for r in rates]
-    if not args.serve_device_only:
-        runs.append(one(rates[0], False))
+    runs = [one(r, True, args.serve_requests) for r in rates]
+    if args.serve_host:
+        runs.append(one(rates[0], False, min(args.serve_requests, 100)))
     return {"workload": "5 Mbp resident genome; Poisson arrivals; request sizes log-uniform in "
                         "1,000-200,000 reads of 36-250 bp; k=ceil(5% L)",
             "requests": args.serve_requests, "slots": args.serve_slots, "runs": runs}
 
 
-def records_leg(job, db, reads, genome, res, ops):
-    """TSV records of every hit of the step just run: srm_job_records (device,
-    including the copy of the text to the host) against srm_format_records on the
-    host cores, byte for byte."""
-    import ctypes as C
-    import saruman_amd as sa
-    from saruman_amd import workload as wl
-    from saruman_amd._lib import ptr
-    lib = sa.load()
-    rid = np.ascontiguousarray(db.read_id, np.int32)
-    pos = np.ascontiguousarray(db.pos, np.int64)
-    text, hits = job.records(pos, read_id=rid, prefix="r")          # first call sizes buffers
-    pbuf = wl.Pinned(len(text) + 1)                                  # page-locked output
-    t0 = time.perf_counter()
-    got, hits = job.records(pos, read_id=rid, prefix="r", out=pbuf.array, copy=False)
-    dev_ms = (time.perf_counter() - t0) * 1e3
-    text = bytes(pbuf.array[:got])
-    pbuf.free()
-    fwd = np.frombuffer(reads.seq, np.uint8)
-    rev = wl.revcomp_all(reads)
-    roff = np.ascontiguousarray(reads.off[db.read_id], np.uint64)
-    g = np.frombuffer(genome.seq, np.uint8)
-    out = np.empty(len(text) + 1, np.uint8)
-    nh = C.c_longlong()
-    threads = 16
-    t0 = time.perf_counter()
-    need = lib.srm_format_records(int(db.n), ptr(res), ptr(ops), ptr(db.cand), ptr(g), ptr(pos),
-                                  ptr(fwd), ptr(rev), ptr(roff), ptr(rid), b"r", ptr(out),
-                                  out.size, C.byref(nh), threads)
-    host_ms = (time.perf_counter() - t0) * 1e3
-    same = need == len(text) and out[:need].tobytes() == text and nh.value == hits
-    return {"device_ms": dev_ms, "host_ms": host_ms,
-            "host_threads": threads, "records": int(hits), "mb": len(text) / 1e6,
-            "identical": bool(same)}
-
-
 def device_filter_leg(ctx, index, genome, reads, chunk, host_strands):
-    """Both strands through the GPU q-gram filter (one untimed pass, one timed),
-    checked field by field against the host filter's output."""
+    """Both strands through the GPU q-gram filter (one untimed pass, one timed);
+    checked field by field against the host filter's output when that ran."""
     from saruman_amd import workload as wl
     df = wl.DeviceFilter(ctx, index, genome)
     preads, buf = wl.pinned_reads(reads)      # bases in page-locked memory
@@ -220,11 +226,41 @@ def device_filter_leg(ctx, index, genome, reads, chunk, host_strands):
         df.close()
         buf.free()
     fields = ("read", "pos", "flag", "w", "batch_end", "perfect_read", "perfect_pos")
-    same = all(np.array_equal(getattr(h, f), getattr(d, f))
-               for h, d in zip(host_strands, out) for f in fields)
-    return {"wall_s": wall, "device_ms": st["device_ms"], "identical": bool(same),
-            "host_s": sum(h.seconds for h in host_strands), "lists": st["lists"],
-            "list_entries": st["list_entries"], "overflow_reads": st["overflow_reads"]}
+    same = None
+    if host_strands is not None:
+        same = bool(all(np.array_equal(getattr(h, f), getattr(d, f))
+                        for h, d in zip(host_strands, out) for f in fields))
+    return out, {"wall_s": wall, "device_ms": st["device_ms"], "identical": same,
+                 "host_s": sum(h.seconds for h in host_strands) if host_strands else None,
+                 "lists": st["lists"], "list_entries": st["list_entries"],
+                 "overflow_reads": st["overflow_reads"]}
+
+
+def host_records(lib, db, reads, lo, genome, res, ops, threads=16):
+    """srm_format_records on the host cores for one batch: the reference's lines
+    for every hit, in candidate order."""
+    import ctypes as C
+    from saruman_amd import workload as wl
+    from saruman_amd._lib import ptr
+    from saruman_amd.workload import subset_reads
+    hi = lo + int(db.read_id.max()) + 1 if db.n else lo
+    sub = subset_reads(reads, lo, hi)
+    fwd = np.frombuffer(sub.seq, np.uint8)
+    rev = wl.revcomp_all(sub)
+    roff = np.ascontiguousarray(sub.off[db.read_id], np.uint64)
+    rid = np.ascontiguousarray(db.read_id + lo, np.int32)
+    g = np.frombuffer(genome.seq, np.uint8)
+    pos = np.ascontiguousarray(db.pos, np.int64)
+    nh = C.c_longlong()
+    need = lib.srm_format_records(int(db.n), ptr(res), ptr(ops), ptr(db.cand), ptr(g), ptr(pos),
+                                  ptr(fwd), ptr(rev), ptr(roff), ptr(rid), b"r", None, 0,
+                                  C.byref(nh), threads)
+    out = np.empty(max(int(need), 1), np.uint8)
+    t0 = time.perf_counter()
+    lib.srm_format_records(int(db.n), ptr(res), ptr(ops), ptr(db.cand), ptr(g), ptr(pos),
+                           ptr(fwd), ptr(rev), ptr(roff), ptr(rid), b"r", ptr(out), out.size,
+                           C.byref(nh), threads)
+    return out[:need].tobytes(), int(nh.value), time.perf_counter() - t0
 
 
 def main():
@@ -234,21 +270,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mixed", choices=sorted(WORKLOAD_TEXT))
     ap.add_argument("--reads", type=int, default=None, help="override the read count")
+    ap.add_argument("--batch-reads", type=int, default=None,
+                    help="reads per batch (default: the workload's; all reads in one batch)")
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=300_000)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-h2h", action="store_true", help="skip the host-to-host pipeline leg")
     ap.add_argument("--no-tuning", action="store_true", help="ignore saruman_amd/tuning/*.json")
     ap.add_argument("--no-dfilter", action="store_true", help="skip the GPU q-gram filter leg")
-    ap.add_argument("--no-records", action="store_true", help="skip the record formatting leg")
+    ap.add_argument("--host-filter", action="store_true",
+                    help="run the host filter too when the device filter makes the candidates")
     ap.add_argument("--class-streams", type=int, default=None,
                     help="run the band classes concurrently on N streams (overrides the tuning)")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rates", default="4,16,64", help="Poisson arrivals per second (sweep)")
-    ap.add_argument("--serve-device-only", action="store_true",
-                    help="skip the host-filter serving run")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rates", default="24,48", help="Poisson arrivals per second (sweep)")
+    ap.add_argument("--serve-host", action="store_true",
+                    help="also serve with the filter and records on the host (100 requests)")
     ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -258,8 +299,11 @@ def main():
         sys.exit(2)
 
     import saruman_amd as sa
+    from saruman_amd import tune
     from saruman_amd import workload as wl
+    from saruman_amd.pipeline import BatchPipeline, PinnedBatch
 
+    lib = sa.load()
     t0 = time.perf_counter()
     cfg, genome, reads = wl.make_workload(args.workload, n_reads=args.reads)
     t_gen = time.perf_counter() - t0
@@ -267,77 +311,123 @@ def main():
     t0 = time.perf_counter()
     index = wl.Index(genome, wl.needed_qs(reads))
     t_index = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    strands = [wl.run_filter(index, genome, reads, dr, cfg["chunk"]) for dr in (0, 4)]
-    t_filter = time.perf_counter() - t0
-    db = wl.device_batch(reads, strands)
-    perfect_reads = np.unique(np.concatenate([s.perfect_read for s in strands]))
-    n_batches = sum(len(s.batch_end) for s in strands)
-    log(f"[bench] index {t_index:.1f}s, filter {t_filter:.1f}s: {db.n} candidates in "
-        f"{n_batches} reference batches, {len(perfect_reads)} reads with perfect hits")
 
+    # candidates: the host filter, and the device filter checked against it; with
+    # the 10M-read workload the device filter makes them (identical by the tests)
+    big = args.workload == "scale"
+    host_strands, t_filter = None, None
+    if not big or args.host_filter:
+        t0 = time.perf_counter()
+        host_strands = [wl.run_filter(index, genome, reads, dr, cfg["chunk"]) for dr in (0, 4)]
+        t_filter = time.perf_counter() - t0
     ctx = sa.Context(0)
     gpu_filter = None
-    if not args.no_dfilter:
-        gpu_filter = device_filter_leg(ctx, index, genome, reads, cfg["chunk"], strands)
+    strands = host_strands
+    if not args.no_dfilter or host_strands is None:
+        dstrands, gpu_filter = device_filter_leg(ctx, index, genome, reads, cfg["chunk"],
+                                                 host_strands)
         log(f"[bench] GPU filter: {gpu_filter['wall_s']:.3f}s for both strands "
             f"(device {gpu_filter['device_ms']:.1f} ms), identical to the host filter: "
             f"{gpu_filter['identical']}")
+        if strands is None:
+            strands = dstrands
     index.close()
-    from saruman_amd import tune
+    perfect_all = np.unique(np.concatenate([s.perfect_read for s in strands]))
+    batch_reads = args.batch_reads or BATCH_READS.get(args.workload, reads.n)
+    batches = wl.batches_by_reads(reads, strands, batch_reads)
+    n_cand = sum(db.n for _, _, db in batches)
+    log(f"[bench] index {t_index:.1f}s, filter {t_filter if t_filter else 0:.1f}s: {n_cand} "
+        f"candidates in {len(batches)} batch(es) of up to {batch_reads} reads, "
+        f"{len(perfect_all)} reads with perfect hits")
+
     tuning = None if args.no_tuning else tune.apply_cached(ctx)
     if args.class_streams is not None:
         tuning = dict(tuning or {"blocks_per_cu": [0] * 9, "merge_below": 0})
         tuning["class_streams"] = args.class_streams
         tune.apply(ctx, tuning)
     log(f"[bench] launch tuning: {tuning if tuning else 'defaults'}")
+
+    # ---- everything resident: genome, and per batch a job with its candidates,
+    # read codes and record inputs; one graph per job (band phase + records)
     t0 = time.perf_counter()
     ctx.set_genome(wl.genome_words(genome))
-    job = sa.Job(ctx)
-    job.upload(0, 1, db.cand, db.read_words, None, ops_total=db.ops_total)
+    jobs = []
+    for lo, hi, db in batches:
+        job = sa.Job(ctx)
+        job.upload(0, 1, db.cand, db.read_words, None, ops_total=db.ops_total)
+        job.set_records(db.pos, read_id=(db.read_id + lo).astype(np.int32), prefix="r")
+        job.use_graph(not args.no_graph)
+        jobs.append(job)
+    lib.srm_device_sync(0)
     t_upload = time.perf_counter() - t0
-    paths = job.paths()
-    log(f"[bench] uploaded in {t_upload:.2f}s; paths banded/plain/bytes = {paths}")
-    lib = sa.load()
+    paths = [j.paths() for j in jobs]
+    log(f"[bench] uploaded {len(jobs)} job(s) in {t_upload:.2f}s; paths banded/plain/bytes = "
+        f"{tuple(int(sum(p[i] for p in paths)) for i in range(3))}")
 
-    # ---- batched persistent kernel: untimed warmup, then exactly K timed steps
-    if not args.no_graph:
-        job.use_graph(True)
-    for _ in range(args.warmup):
+    # reads mapped per batch (from one run of each job)
+    mapped = []
+    hits_total = 0
+    aligned_reads = 0
+    for (lo, hi, db), job in zip(batches, jobs):
         job.run("banded")
+        res, _ = job.download()
+        hit = res["score"].astype(np.int64) <= db.k
+        hits_total += int(hit.sum())
+        al = np.unique(db.read_id[hit])
+        aligned_reads += len(al)
+        mapped.append(len(np.union1d(al, db.extra["perfect_read"])))
+    n_mapped = int(sum(mapped))
+
+    # ---- untimed warmup, then exactly K timed steps (step i runs batch i mod B)
+    B = len(jobs)
+    for i in range(args.warmup):
+        jobs[i % B].run("banded")
     lib.srm_device_sync(0)
     t0 = time.perf_counter()
     dev_ms = []
-    for _ in range(args.steps):
-        dev_ms.append(job.run("banded"))
+    reads_done = 0
+    for i in range(args.steps):
+        dev_ms.append(jobs[i % B].run("banded"))
+        reads_done += mapped[i % B]
     lib.srm_device_sync(0)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    # event-timed band kernels (non-graph launch, same stream)
-    job.use_graph(False)
-    job.run("banded")
-    band_ms, _ = job.kernel_ms()
-    t0 = time.perf_counter()
-    res, ops = job.download()
-    t_d2h = time.perf_counter() - t0
-    hit = res["score"].astype(np.int64) <= db.k
-    mapped = np.union1d(np.unique(db.read_id[hit]), perfect_reads)
-    n_mapped = len(mapped)
-    log(f"[bench] banded: {ms_per_step:.3f} ms/step (device {np.median(dev_ms):.3f} ms, "
-        f"band kernels {band_ms:.3f} ms), hits {int(hit.sum())}, reads mapped {n_mapped}")
+    value = reads_done / wall
 
-    # ---- records of every hit: formatted on the device vs on 16 host threads
-    records = None
-    if not args.no_records:
-        records = records_leg(job, db, reads, genome, res, ops)
-        log(f"[bench] records: device {records['device_ms']:.1f} ms (incl. the copy to the host), "
-            f"host {records['host_ms']:.1f} ms on {records['host_threads']} threads, "
-            f"{records['mb']:.0f} MB, identical: {records['identical']}")
+    # ---- each job once more without the graph: per-stage HIP-event times; the
+    # record text against the host formatter on the same results
+    band_ms, rec_ms, identical = [], [], True
+    text_bytes = 0
+    t_host_fmt = 0.0
+    check = range(B) if B <= 3 else (0, B - 1)
+    for b, ((lo, hi, db), job) in enumerate(zip(batches, jobs)):
+        job.use_graph(False)
+        job.run("banded")
+        band_ms.append(job.kernel_ms()[0])
+        rec_ms.append(float(lib.srm_job_kernel_ms(job.h, 2)))
+        text, nh = job.records_text()
+        text_bytes += len(text)
+        if b in check:
+            res, ops = job.download()
+            host_text, host_hits, secs = host_records(lib, db, reads, lo, genome, res, ops)
+            t_host_fmt += secs
+            identical = identical and host_text == text and host_hits == nh
+        job.use_graph(not args.no_graph)
+    band_ms_step = float(np.mean(band_ms))
+    rec_ms_step = float(np.mean(rec_ms))
+    log(f"[bench] step: {ms_per_step:.3f} ms (graph replay; band phase {band_ms_step:.3f} ms, "
+        f"records {rec_ms_step:.3f} ms by events), {n_mapped} reads mapped, {hits_total} hits; "
+        f"device records == host records: {identical}")
 
-    # ---- plain kernel on the same job
+    # ---- the plain kernel on the first batch (the same record stage)
     plain = None
     if not args.no_plain:
-        job.run("plain")   # one untimed pass (first-launch costs)
+        job = jobs[0]
+        job.use_graph(False)
+        job.run("banded")
+        bres, bops = job.download()
+        btext, _ = job.records_text()
+        job.run("plain")   # one untimed pass (first-launch costs, matrix allocation)
         lib.srm_device_sync(0)
         t0 = time.perf_counter()
         for _ in range(args.plain_steps):
@@ -345,36 +435,69 @@ def main():
         lib.srm_device_sync(0)
         p_ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
         pres, pops = job.download()
-        same = bool((np.minimum(pres["score"], db.k + 1) == np.minimum(res["score"], db.k + 1)).all()
-                    and (pres["n_ops"] == res["n_ops"]).all()
-                    and (pres["end_row"] == res["end_row"]).all()
-                    and (pres["end_col"] == res["end_col"]).all()
-                    and (pops == ops).all())
-        plain = dict(ms_per_step=p_ms, value=n_mapped / (p_ms / 1e3), results_identical=same)
-        log(f"[bench] plain: {p_ms:.1f} ms/step, identical results: {same}")
-    job.close()
+        ptext, _ = job.records_text()
+        db0 = batches[0][2]
+        same = bool((np.minimum(pres["score"], db0.k + 1) == np.minimum(bres["score"], db0.k + 1)).all()
+                    and (pres["n_ops"] == bres["n_ops"]).all()
+                    and (pres["end_row"] == bres["end_row"]).all()
+                    and (pres["end_col"] == bres["end_col"]).all()
+                    and ptext == btext)
+        job.use_graph(not args.no_graph)
+        # the banded step time of the same batch
+        b_ms = ms_per_step if B == 1 else float(np.median([dev_ms[i] for i in range(0, args.steps, B)]))
+        plain = dict(ms_per_step=p_ms, value=mapped[0] / (p_ms / 1e3), results_identical=same,
+                     banded_ms_same_batch=b_ms)
+        log(f"[bench] plain: {p_ms:.1f} ms/step for batch 0, identical results: {same}")
 
-    L = db.L.astype(np.float64)
-    W = db.w.astype(np.float64)
-    K = db.k.astype(np.float64)
-    ref_cells = float((L * W).sum())
-    bcells = float(band_cells(L, K, db.flag).sum())
-    achieved = OPS_PER_BAND_CELL * bcells / (band_ms / 1e3) / 1e12
-    hbm = float(hbm_bytes(L, W, K).sum())
+    # ---- host to host: batches from page-locked memory through two lanes, records
+    # copied back, transfers overlapped with the other lane's device work
+    h2h = None
+    if not args.no_h2h:
+        reps = max(1, 3 // B) if B < 3 else 1
+        pinned = [PinnedBatch.of(db, lo) for lo, hi, db in batches]
+        order = [pinned[i % B] for i in range(B * reps)]
+        with BatchPipeline(wl.genome_words(genome), lanes=2) as pipe:
+            pipe.run(order[:min(2, len(order))])            # untimed: lane buffers, graphs
+            h_wall, h_res, _ = pipe.run(order)
+        for p in pinned:
+            p.free()
+        h_reads = sum(mapped[i % B] for i in range(B * reps))
+        h2h = {"batches": len(order), "wall_s": h_wall, "reads_mapped_per_s": h_reads / h_wall,
+               "text_bytes": int(sum(r.text_bytes for r in h_res)),
+               "mean_upload_ms": float(np.mean([r.upload_s for r in h_res])) * 1e3,
+               "mean_step_ms": float(np.mean([r.step_ms for r in h_res])),
+               "mean_copy_out_ms": float(np.mean([r.copy_s for r in h_res])) * 1e3,
+               "lanes": 2}
+        log(f"[bench] host to host: {len(order)} batches in {h_wall:.3f} s -> "
+            f"{h2h['reads_mapped_per_s']:.0f} reads mapped/s (upload {h2h['mean_upload_ms']:.1f} ms, "
+            f"step {h2h['mean_step_ms']:.2f} ms, copy-out {h2h['mean_copy_out_ms']:.1f} ms per batch)")
+
+    for job in jobs:
+        job.close()
+
+    L = np.concatenate([db.L for _, _, db in batches]).astype(np.float64)
+    W = np.concatenate([db.w for _, _, db in batches]).astype(np.float64)
+    K = np.concatenate([db.k for _, _, db in batches]).astype(np.float64)
+    F = np.concatenate([db.flag for _, _, db in batches])
+    steps_per_pass = B
+    ref_cells = float((L * W).sum()) / steps_per_pass      # per step (average batch)
+    bcells = float(band_cells(L, K, F).sum()) / steps_per_pass
+    achieved = OPS_PER_BAND_CELL * bcells / (band_ms_step / 1e3) / 1e12
+    hbm = float(hbm_bytes(L, W, K).sum()) / steps_per_pass
 
     cpu = None
     if not args.no_cpu:
-        cpu, _ = cpu_baseline(genome, reads, db, args.cpu_sample, n_mapped / max(db.n, 1))
+        lo0, _, db0 = batches[0]
+        cpu = cpu_baseline(genome, reads, lo0, db0, args.cpu_sample, args.cpu_runs,
+                           mapped[0] / max(db0.n, 1))
         log(f"[bench] cpu oracle: {cpu['candidates_per_s']:.0f} candidates/s on {cpu['cores']} threads")
 
     serving = None
     if not args.no_serve:
         serving = serve_leg(args)
 
-    value = n_mapped / (ms_per_step / 1e3)
     out = {
-        "metric": "reads mapped/sec (batched persistent banded kernel; GCUPS and the plain "
-                  "kernel alongside)",
+        "metric": METRIC,
         "value": value,
         "unit": "reads/s",
         "n_gpus": 1,
@@ -387,29 +510,35 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}",
-                   "reads": int(reads.n), "candidates": int(db.n), "reads_mapped": int(n_mapped),
-                   "hits": int(hit.sum()), "match": 0, "mismatch": 1, "parallelism": "1 GPU",
-                   "graph": not args.no_graph,
-                   "tuning": tuning_key(tuning)},
+                   "reads": int(reads.n), "candidates": int(n_cand), "batches": B,
+                   "reads_per_batch": int(batch_reads), "reads_mapped": n_mapped,
+                   "reads_mapped_aligned": int(aligned_reads),
+                   "reads_mapped_perfect_only": int(n_mapped - aligned_reads),
+                   "hits": hits_total, "match": 0, "mismatch": 1, "parallelism": "1 GPU",
+                   "step": "band kernels + record kernels, candidates and records in HBM",
+                   "graph": not args.no_graph, "tuning": tuning_key(tuning)},
+        "stages_ms": {"band": band_ms_step, "records": rec_ms_step},
+        "band_phase_reads_per_s": n_mapped / B / (band_ms_step / 1e3),
         "gcups": ref_cells / (ms_per_step / 1e3) / 1e9,
-        "band_gcups": bcells / (band_ms / 1e3) / 1e9,
+        "band_gcups": bcells / (band_ms_step / 1e3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS,
-                     "traffic": profiled_traffic(args.workload, int(db.n), tuning),
+                     "traffic": profiled_traffic(args.workload, int(n_cand), tuning),
                      "kernel": "bits_band_kernel + bits_trace_kernel of every band class of one "
-                               "step (HIP events around the band phase)",
+                               "step (HIP events around the band phase); the dominant stage",
                      "ops_per_step": OPS_PER_BAND_CELL * bcells,
-                     "kernel_ms": band_ms},
-        "roofline_hbm": {"achieved": hbm / (band_ms / 1e3) / 1e9, "peak": PEAK_HBM_GBS,
+                     "kernel_ms": band_ms_step},
+        "roofline_hbm": {"achieved": hbm / (band_ms_step / 1e3) / 1e9, "peak": PEAK_HBM_GBS,
                          "unit": "GB/s", "algorithmic_bytes": hbm},
+        "records": {"text_bytes_per_pass": text_bytes, "device_equals_host": bool(identical),
+                    "host_format_s": t_host_fmt, "host_threads": 16},
         "plain": plain,
-        "speedup_vs_plain": (plain["ms_per_step"] / ms_per_step) if plain else None,
+        "speedup_vs_plain": (plain["ms_per_step"] / plain["banded_ms_same_batch"]) if plain else None,
         "cpu_baseline": cpu,
+        "host_to_host": h2h,
         "host": {"generate_s": t_gen, "index_s": t_index, "filter_s": t_filter,
-                 "upload_s": t_upload, "download_s": t_d2h},
-        "e2e_reads_per_s_pcie": n_mapped / (t_upload + ms_per_step / 1e3 + t_d2h),
+                 "upload_s": t_upload},
         "gpu_filter": gpu_filter,
-        "records": records,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

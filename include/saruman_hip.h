@@ -294,7 +294,8 @@ int srm_job_path_count(srm_job *job, int path);
  * post-processing and printf of src/cuda-functions.cu:828-994 for each candidate
  * with score <= k.  pos[n]: the window's genome position (record start before
  * leading gaps).  Names: with a table, names[read_id[t] * name_stride ...] (NUL
- * terminated within the stride); without, name_prefix + decimal read_id[t]
+ * terminated within the stride, name_stride <= 256); without, name_prefix (at
+ * most 255 bytes) + decimal read_id[t]
  * (read_id NULL: the candidate index).  Returns the text's byte count; the text
  * is written to out only when outcap >= that count (a size query otherwise), or
  * an SRM_E* code (< 0).  n_hits (may be NULL) receives the number of records.

@@ -999,8 +999,8 @@ int rec_inputs(srm_job *j, const int64_t *pos, const int32_t *read_id, const cha
     srm_ctx *ctx = j->ctx;
     const int n = j->n;
     if (names) {
-        if (name_stride < 2 || !read_id)
-            return fail(SRM_EINVAL, "a name table needs name_stride >= 2 and read ids");
+        if (name_stride < 2 || name_stride > 256 || !read_id)
+            return fail(SRM_EINVAL, "a name table needs 2 <= name_stride <= 256 and read ids");
         for (int t = 0; t < n; ++t)
             if (read_id[t] < 0 || (size_t)read_id[t] >= n_names)
                 return fail(SRM_EINVAL, "read id %d of candidate %d is outside the name table",

@@ -132,3 +132,22 @@ def test_record_stage_in_the_launch_sequence(orc, graph):
     finally:
         job.close()
         other.close()
+
+
+def test_device_records_short_and_long_lines_side_by_side(orc):
+    """Lines of 36-860 bp reads in one batch: the short ones are written in runs by
+    the block kernel, the long ones (over 512 printed positions or 1 KiB of text)
+    by the wave kernel, into neighbouring bytes of the same output; plus names
+    from a table and a prefix, and raw-byte candidates in between."""
+    print("[gpu] device records: short and long lines", flush=True)
+    rng = np.random.default_rng(25)
+    batch = random_batch(rng, 1500, L=(36, 860), k=14, flags=(1, 3, 5, 7),
+                         per_cand_k=lambda L: min(14, math.ceil(0.03 * L)))
+    for t in range(0, batch.n, 7):
+        wl = bytearray(batch.windows[t])
+        wl[int(rng.integers(0, len(wl)))] = ord("N")
+        batch.windows[t] = bytes(wl)
+    want = orc.align_batch(batch)
+    got = sa.align_batch(batch, device_records=True)
+    assert got.paths[2] > 0
+    check_text(batch, got, want)
