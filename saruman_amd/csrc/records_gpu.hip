@@ -142,14 +142,12 @@ __global__ void __launch_bounds__(RB) rec_measure(RecArgs a)
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(a.hits, (uint32_t)__popcll(b));
 }
 
-// run sizes of rec_write_group (below), which writes the lines grouped_line() accepts
-constexpr int GB = 256;              // threads per block = candidates per tile
-constexpr int GL = 16;               // lines per run
-constexpr int TPL = GB / GL;         // threads per line
-constexpr int CHUNK = 32;            // printed positions per thread
+// line sizes of rec_write_quad (below), which writes the lines grouped_line() accepts
+constexpr int QL = 4;                // lines per wave
+constexpr int TPL = 16;              // lanes per line
+constexpr int CHUNK = 32;            // printed positions per lane at most
 constexpr int MAXP = TPL * CHUNK;    // printed positions per line (512)
 constexpr int LCAP = 1024;           // text bytes per line
-constexpr int GTEXT = GL * LCAP;     // text bytes per run
 constexpr int WPL = MAXP / 16 + 3;   // staged words per line: transcript, read codes, window codes
 constexpr int NAMEW = 64;            // staged name words per line (name_stride <= 256)
 
@@ -239,7 +237,7 @@ __global__ void __launch_bounds__(RB) rec_write(RecArgs a)
             rw_l[0] = rp[0];
             rw_l[1] = rp[1];
         }
-        // the lines rec_write_group leaves: long ones and raw-byte candidates
+        // the lines rec_write_quad leaves: long ones and raw-byte candidates
         uint64_t hits = __ballot(len_l != 0 && !grouped_line(len_l, inf_l.z, cw_l[4] >> 16));
         if (!hits) continue;
         const uint64_t off_l = len_l ? a.off[tl] : 0;
@@ -395,20 +393,22 @@ __global__ void __launch_bounds__(RB) rec_write(RecArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// rec_write_group: the lines of up to 16 consecutive hits at a time per block.
-// A block takes 256 candidates (one per thread: its length, offset and metadata in
-// one coalesced load per field) and cuts its hits, in candidate order, into runs
-// of up to GL lines whose text is contiguous in the output.  For each run:
-//   stage   the transcript words and the read and window code words each line's
-//           rows need, and a name from the table, into LDS (TPL threads per line,
-//           every load of the run in flight at once)
-//   build   header, rows and tail into one LDS text buffer at the output's byte
-//           alignment; a line's rows in chunks of CHUNK printed positions per
-//           thread, placed by a scan of the chunks' row and column advances
-//   copy    the run's text to the output with dword stores (bytes only where the
-//           first and last dwords are shared with the neighbouring text)
-// Lines longer than LCAP bytes or MAXP printed positions, and raw-byte
-// candidates, are left to rec_write, which writes only those.
+// rec_write_quad: four lines per wave at a time, TPL = 16 lanes per line.  A wave
+// takes 64 consecutive candidates (their length, offset and metadata in one
+// coalesced load per field, a candidate per lane) and writes the lines
+// grouped_line() accepts, four at a time:
+//   stage   each line's lanes load the transcript words and the read and window
+//           code words its rows need (and a name from the table) into the line's
+//           LDS slot: the four lines' loads are in flight together
+//   build   header, rows and tail into the slot at the output's byte alignment; the
+//           rows split evenly over the line's lanes (at most CHUNK printed positions
+//           each), placed by a scan of the runs' row and column advances
+//   copy    the line's lanes store it with dwords, bytes only in the first and last
+//           dwords, which the neighbouring lines share
+// Every step of a line stays within its wave: no block barrier.  Long lines and
+// raw-byte candidates are rec_write's.
+constexpr int SLOT_TEXT = LCAP / 4 + 1;    // text words per slot (a line at its alignment)
+
 // 2-bit field idx (0..47) of three consecutive words
 __device__ __forceinline__ uint32_t bits2(uint32_t w0, uint32_t w1, uint32_t w2, int idx)
 {
@@ -416,156 +416,113 @@ __device__ __forceinline__ uint32_t bits2(uint32_t w0, uint32_t w1, uint32_t w2,
     return (w >> ((idx & 15) * 2)) & 3u;
 }
 
-// per-line metadata of a run, in LDS
-enum { M_ROFF, M_WOFF, M_OPSOFF, M_LW, M_FLAG, M_RES0, M_RES1, M_START, M_STOP, M_LEADRN,
-       M_OFFLO, M_OFFHI, M_ID, M_LEN, M_WORDS };
-
-__global__ void __launch_bounds__(GB) rec_write_group(RecArgs a)
+__device__ __forceinline__ uint32_t from_lane(uint32_t v, int src)
 {
-    __shared__ uint32_t s_text[GTEXT / 4 + 2];
-    __shared__ uint32_t s_ops[GL][WPL], s_rc[GL][WPL], s_gc[GL][WPL];
-    __shared__ uint32_t s_name[GL][NAMEW];
-    __shared__ uint32_t s_meta[GL][M_WORDS];
-    __shared__ int s_list[GB];             // the tile's hits, candidate order: thread, -1 - thread if not grouped
-    __shared__ uint32_t s_hlen[GB];        // their text lengths
-    __shared__ int s_slot[GB];             // per thread: run << 8 | line in run, or -1
-    __shared__ int s_rstart[GB], s_rcount[GB];
-    __shared__ int s_nruns, s_nhits;
-    __shared__ int s_wsum[GB / 64];
-    __shared__ char s_prefix[256];
+    return (uint32_t)__shfl((int)v, src, 64);
+}
 
+__device__ __forceinline__ uint64_t bitrange64(int lo, int hi)   // bits [lo, hi) of 0..63
+{
+    if (hi <= lo) return 0ull;
+    const uint64_t top = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
+    return top & ~((1ull << lo) - 1ull);
+}
+
+__global__ void __launch_bounds__(RB) rec_write_quad(RecArgs a)
+{
+    __shared__ uint32_t s_text[RB / 64][QL][SLOT_TEXT];
+    __shared__ uint32_t s_ops[RB / 64][QL][WPL], s_rc[RB / 64][QL][WPL], s_gc[RB / 64][QL][WPL];
+    __shared__ uint32_t s_name[RB / 64][QL][NAMEW];
+    __shared__ char s_prefix[256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int q = lane / TPL, u = lane % TPL;      // this lane's line slot and place in it
     if (!a.names)
-        for (int q = tid; q < a.prefix_len; q += GB) s_prefix[q] = a.prefix[q];
-    const int ntile = (a.n + GB - 1) / GB;
-    for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        // ---- the tile's candidates, one per thread
-        const int t = tile * GB + tid;
-        const bool in = t < a.n;
-        const uint64_t len = in ? a.len[t] : 0;
-        const bool hit = len > 0;
+        for (int i = tid; i < a.prefix_len; i += RB) s_prefix[i] = a.prefix[i];
+    __syncthreads();
+    const int nwaves = gridDim.x * (RB / 64);
+    const int ntile = (a.n + 63) / 64;
+    const uint64_t e64 = 0x5555555555555555ull;
+    for (int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * (RB / 64) + wv); tile < ntile;
+         tile += nwaves) {
+        // ---- the tile's candidates, one per lane
+        const int tl = tile * 64 + lane;
+        const bool in = tl < a.n;
+        const uint64_t len_l = in ? a.len[tl] : 0;
+        if (!__ballot(len_l != 0)) continue;
         uint32_t cw[5] = {0u, 0u, 0u, 0u, 0u}, rw[2] = {0u, 0u};
         uint4 inf = make_uint4(0u, 0u, 0u, 0u);
-        uint64_t off = 0;
-        int32_t id = 0;
-        if (hit) {
-            const uint32_t *cp = (const uint32_t *)(a.cand + t);
-            const uint32_t *rp = (const uint32_t *)(a.res + t);
+        uint64_t off_l = 0;
+        int32_t id_l = 0;
+        if (len_l) {
+            const uint32_t *cp = (const uint32_t *)(a.cand + tl);
+            const uint32_t *rp = (const uint32_t *)(a.res + tl);
             for (int i = 0; i < 5; ++i) cw[i] = cp[i];
             rw[0] = rp[0];
             rw[1] = rp[1];
-            inf = a.info[t];
-            off = a.off[t];
-            id = a.read_id ? a.read_id[t] : t;
+            inf = a.info[tl];
+            off_l = a.off[tl];
+            id_l = a.read_id ? a.read_id[tl] : tl;
         }
-        const bool grp = hit && grouped_line(len, inf.z, cw[4] >> 16);
-        // ---- compact the hits in candidate order
-        const uint64_t hb = __ballot(hit);
-        const int before = __popcll(hb & ((1ull << lane) - 1ull));
-        if (lane == 0) s_wsum[wv] = __popcll(hb);
-        __syncthreads();
-        int base = 0, nh = 0;
-        for (int x = 0; x < GB / 64; ++x) {
-            if (x < wv) base += s_wsum[x];
-            nh += s_wsum[x];
-        }
-        const int kk = base + before;
-        if (hit) {
-            s_list[kk] = grp ? tid : -1 - tid;
-            s_hlen[kk] = (uint32_t)len;
-        }
-        s_slot[tid] = -1;
-        __syncthreads();
-        // ---- runs of consecutive grouped hits (at most GL lines, GTEXT - 4 bytes)
-        if (tid == 0) {
-            int nr = 0, k = 0;
-            while (k < nh) {
-                if (s_list[k] < 0) {
-                    ++k;
-                    continue;
-                }
-                s_rstart[nr] = k;
-                int m = 0;
-                uint32_t tot = 0;
-                while (k < nh && s_list[k] >= 0 && m < GL && tot + s_hlen[k] <= (uint32_t)(GTEXT - 4)) {
-                    tot += s_hlen[k];
-                    s_slot[s_list[k]] = (nr << 8) | m;
-                    ++m;
-                    ++k;
-                }
-                s_rcount[nr] = m;
-                ++nr;
+        uint64_t todo = __ballot(len_l != 0 && grouped_line(len_l, inf.z, cw[4] >> 16));
+        while (todo) {
+            // the next QL lines: the owner lane of this lane's slot
+            int owner = -1;
+            for (int g = 0; g < QL && todo; ++g) {
+                const int h = (int)__builtin_ctzll(todo);
+                todo &= todo - 1;
+                if (g == q) owner = h;
             }
-            s_nruns = nr;
-            s_nhits = nh;
-        }
-        __syncthreads();
-        const int nruns = s_nruns;
-        const int myslot = s_slot[tid];
-        for (int g = 0; g < nruns; ++g) {
-            // ---- the run's metadata from its owners
-            if (grp && (myslot >> 8) == g) {
-                uint32_t *m = s_meta[myslot & 0xFF];
-                m[M_ROFF] = cw[0];
-                m[M_WOFF] = cw[1];
-                m[M_OPSOFF] = cw[2];
-                m[M_LW] = cw[3];
-                m[M_FLAG] = cw[4];
-                m[M_RES0] = rw[0];
-                m[M_RES1] = rw[1];
-                m[M_START] = inf.x;
-                m[M_STOP] = inf.y;
-                m[M_LEADRN] = inf.z;
-                m[M_OFFLO] = (uint32_t)off;
-                m[M_OFFHI] = (uint32_t)(off >> 32);
-                m[M_ID] = (uint32_t)id;
-                m[M_LEN] = (uint32_t)len;
-            }
-            __syncthreads();
-            const int nl = s_rcount[g];
-            const int li = tid / TPL, u = tid % TPL;
-            const bool act = li < nl;
-            const uint32_t *m = s_meta[act ? li : 0];
-            const uint32_t read_off = m[M_ROFF], win_off = m[M_WOFF];
-            const int L = (int)(m[M_LW] & 0xFFFFu), w = (int)(m[M_LW] >> 16);
-            const int flag = (int)(m[M_FLAG] & 0xFFu);
-            const int score = (int)(int16_t)(m[M_RES0] & 0xFFFFu), n = (int)(m[M_RES0] >> 16);
-            const int end_row = (int)(m[M_RES1] & 0xFFFFu), end_col = (int)(m[M_RES1] >> 16);
-            const int lead = (int)(m[M_LEADRN] & 0xFFFFu), rn = (int)(m[M_LEADRN] >> 16);
+            const bool act = owner >= 0;
+            const int src = act ? owner : 0;
+            // the line's metadata from its owner
+            const uint32_t read_off = from_lane(cw[0], src), win_off = from_lane(cw[1], src);
+            const uint32_t ops_off = from_lane(cw[2], src), lw = from_lane(cw[3], src);
+            const uint32_t fl = from_lane(cw[4], src);
+            const uint32_t r0w = from_lane(rw[0], src), r1w = from_lane(rw[1], src);
+            const int32_t start = (int32_t)from_lane(inf.x, src), stop = (int32_t)from_lane(inf.y, src);
+            const uint32_t leadrn = from_lane(inf.z, src);
+            const uint64_t loff = (uint64_t)from_lane((uint32_t)off_l, src) |
+                                  ((uint64_t)from_lane((uint32_t)(off_l >> 32), src) << 32);
+            const uint32_t llen = from_lane((uint32_t)len_l, src);
+            const int32_t lid = (int32_t)from_lane((uint32_t)id_l, src);
+            const int L = (int)(lw & 0xFFFFu), w = (int)(lw >> 16);
+            const int flag = (int)(fl & 0xFFu);
+            const int score = (int)(int16_t)(r0w & 0xFFFFu), n = (int)(r0w >> 16);
+            const int end_row = (int)(r1w & 0xFFFFu), end_col = (int)(r1w >> 16);
+            const int lead = (int)(leadrn & 0xFFFFu), rn = (int)(leadrn >> 16);
             const int endp = lead + rn;
-            const int32_t lid = (int32_t)m[M_ID];
-            const uint64_t loff = (uint64_t)m[M_OFFLO] | ((uint64_t)m[M_OFFHI] << 32);
             const int wlo = endp > 0 ? (n - endp) >> 4 : 0;
+            const int nw_ops = endp > 0 ? ((n - 1) >> 4) - wlo + 1 : 0;
             const int rlo = (int)((read_off + (uint32_t)end_col) >> 4);
             const int glo = (int)((win_off + (uint32_t)end_row) >> 4);
-            // ---- stage: transcript, codes and name of every line of the run
+            const int rnw = (int)((read_off + (uint32_t)min(end_col + endp, L)) >> 4) - rlo + 1;
+            const int gnw = (int)((win_off + (uint32_t)min(end_row + endp, w)) >> 4) - glo + 1;
+            uint32_t *sops = s_ops[wv][q], *src_ = s_rc[wv][q], *sgc = s_gc[wv][q];
+            // ---- stage
             if (act) {
                 if (endp > 0) {
-                    const uint32_t *ops = a.ops + m[M_OPSOFF];
-                    const int nw_ops = ((n - 1) >> 4) - wlo + 1;
-                    for (int k = u; k < nw_ops; k += TPL) s_ops[li][k] = ops[wlo + k];
-                    const int rhi = (int)((read_off + (uint32_t)min(end_col + endp, L)) >> 4);
-                    for (int k = u; k <= rhi - rlo; k += TPL) s_rc[li][k] = a.read_words[rlo + k];
-                    const int ghi = (int)((win_off + (uint32_t)min(end_row + endp, w)) >> 4);
-                    for (int k = u; k <= ghi - glo; k += TPL) s_gc[li][k] = a.win_words[glo + k];
+                    const uint32_t *ops = a.ops + ops_off;
+                    for (int k = u; k < nw_ops; k += TPL) sops[k] = ops[wlo + k];
+                    for (int k = u; k < rnw; k += TPL) src_[k] = a.read_words[rlo + k];
+                    for (int k = u; k < gnw; k += TPL) sgc[k] = a.win_words[glo + k];
                 }
                 if (a.names) {
                     const char *nm = a.names + (int64_t)lid * a.name_stride;
-                    char *sn = (char *)s_name[li];
-                    for (int q = u; q < a.name_stride; q += TPL) sn[q] = nm[q];
+                    char *sn = (char *)s_name[wv][q];
+                    for (int i = u; i < a.name_stride; i += TPL) sn[i] = nm[i];
                 }
             }
-            __syncthreads();
-            // ---- build the run's text in LDS
-            const uint64_t off0 = (uint64_t)s_meta[0][M_OFFLO] | ((uint64_t)s_meta[0][M_OFFHI] << 32);
-            const int al0 = (int)(off0 & 3u);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // ---- build
+            const int al = (int)(loff & 3u);
+            char *line = (char *)s_text[wv][q] + al;
             if (act) {
-                char *line = (char *)s_text + al0 + (int)(loff - off0);
-                const int32_t start = (int32_t)m[M_START], stop = (int32_t)m[M_STOP];
                 const int nds = ndig(start), ndt = ndig(stop), ndi = ndig(lid), ndsc = ndig(score);
                 int name_n;
                 if (a.names) {
-                    const char *sn = (const char *)s_name[li];
+                    const char *sn = (const char *)s_name[wv][q];
                     name_n = 0;
                     while (name_n < a.name_stride - 1 && sn[name_n]) ++name_n;
                 } else {
@@ -574,21 +531,21 @@ __global__ void __launch_bounds__(GB) rec_write_group(RecArgs a)
                 const int hdr = name_n + 1 + nds + 1 + ndt + 1 + 2 + 1;
                 for (int h = u; h < hdr; h += TPL) {
                     char ch;
-                    int q = h;
-                    if (q < name_n) {
+                    int x = h;
+                    if (x < name_n) {
                         if (a.names)
-                            ch = ((const char *)s_name[li])[q];
+                            ch = ((const char *)s_name[wv][q])[x];
                         else
-                            ch = q < a.prefix_len ? s_prefix[q] : digit_at(lid, ndi, q - a.prefix_len);
-                    } else if ((q -= name_n) == 0) {
+                            ch = x < a.prefix_len ? s_prefix[x] : digit_at(lid, ndi, x - a.prefix_len);
+                    } else if ((x -= name_n) == 0) {
                         ch = '\t';
-                    } else if ((q -= 1) < nds) {
-                        ch = digit_at(start, nds, q);
-                    } else if ((q -= nds) == 0) {
+                    } else if ((x -= 1) < nds) {
+                        ch = digit_at(start, nds, x);
+                    } else if ((x -= nds) == 0) {
                         ch = '\t';
-                    } else if ((q -= 1) < ndt) {
-                        ch = digit_at(stop, ndt, q);
-                    } else if ((q -= ndt) == 0 || q == 3) {
+                    } else if ((x -= 1) < ndt) {
+                        ch = digit_at(stop, ndt, x);
+                    } else if ((x -= ndt) == 0 || x == 3) {
                         ch = '\t';
                     } else {
                         ch = flag > 3 ? '<' : '>';
@@ -597,27 +554,33 @@ __global__ void __launch_bounds__(GB) rec_write_group(RecArgs a)
                 }
                 char *orow = line + hdr;
                 char *grow = orow + rn + 1;
-                // rows: positions p0 .. p0 + cnt - 1 of this thread; position p holds op n-1-p
-                const int p0 = u * CHUNK;
-                const int cnt = endp - p0 < 0 ? 0 : (endp - p0 > CHUNK ? CHUNK : endp - p0);
+                // the line's positions split evenly over its lanes (at most CHUNK each)
+                const int chunk = (endp + TPL - 1) / TPL;
+                const int p0 = u * chunk;
+                const int cnt = endp - p0 < 0 ? 0 : (endp - p0 > chunk ? chunk : endp - p0);
                 uint32_t o0 = 0, o1 = 0, o2 = 0;
                 int jb = 0;
-                if (cnt > 0) {
-                    const int j0 = n - 1 - p0, jm = j0 - (cnt - 1);
-                    jb = jm >> 4;
-                    const int nw_ops = ((n - 1) >> 4) - wlo + 1;
-                    const int x = jb - wlo;
-                    o0 = s_ops[li][x];
-                    o1 = x + 1 < nw_ops ? s_ops[li][x + 1] : 0u;
-                    o2 = x + 2 < nw_ops ? s_ops[li][x + 2] : 0u;
-                }
                 uint32_t drow = 0, dcol = 0;
-                for (int i = 0; i < cnt; ++i) {
-                    const uint32_t op = bits2(o0, o1, o2, n - 1 - (p0 + i) - 16 * jb);
-                    drow += op != SRM_OP_LEFT;
-                    dcol += op != SRM_OP_TOP;
+                if (cnt > 0) {
+                    const int jm = n - 1 - p0 - (cnt - 1);
+                    jb = jm >> 4;
+                    const int x = jb - wlo;
+                    o0 = sops[x];
+                    o1 = x + 1 < nw_ops ? sops[x + 1] : 0u;
+                    o2 = x + 2 < nw_ops ? sops[x + 2] : 0u;
+                    // advances of the run: ops other than LEFT move the row, ops other
+                    // than TOP the column, counted over its 2-bit fields [jm, jm + cnt)
+                    const int b0 = 2 * (jm - 16 * jb), b1 = b0 + 2 * cnt;
+                    const uint64_t X = (uint64_t)o0 | ((uint64_t)o1 << 32);
+                    const uint64_t mx = bitrange64(b0 < 64 ? b0 : 64, b1 < 64 ? b1 : 64);
+                    const uint64_t my = bitrange64(b0 > 64 ? b0 - 64 : 0, b1 > 64 ? b1 - 64 : 0);
+                    const uint64_t xl = X & e64, xh = (X >> 1) & e64;
+                    const uint64_t yl = (uint64_t)o2 & e64, yh = ((uint64_t)o2 >> 1) & e64;
+                    const int tops = __popcll(xl & ~xh & mx) + __popcll(yl & ~yh & my);
+                    const int lefts = __popcll(xh & ~xl & mx) + __popcll(yh & ~yl & my);
+                    drow = (uint32_t)(cnt - lefts);
+                    dcol = (uint32_t)(cnt - tops);
                 }
-                // exclusive scan of the advances over the line's TPL threads
                 uint32_t sr = drow, sc = dcol;
                 for (int d = 1; d < TPL; d <<= 1) {
                     const uint32_t yr = __shfl_up(sr, d, TPL), yc = __shfl_up(sc, d, TPL);
@@ -626,25 +589,23 @@ __global__ void __launch_bounds__(GB) rec_write_group(RecArgs a)
                         sc += yc;
                     }
                 }
-                int row = end_row + (int)(sr - drow), col = end_col + (int)(sc - dcol);
+                const int row = end_row + (int)(sr - drow), col = end_col + (int)(sc - dcol);
                 if (cnt > 0) {
-                    const int rnw = (int)((read_off + (uint32_t)min(end_col + endp, L)) >> 4) - rlo + 1;
-                    const int gnw = (int)((win_off + (uint32_t)min(end_row + endp, w)) >> 4) - glo + 1;
                     const int ri = (int)(read_off + (uint32_t)col) - 16 * rlo;
                     const int gi = (int)(win_off + (uint32_t)row) - 16 * glo;
                     const int rx = ri >> 4, gx = gi >> 4;
-                    const uint32_t r0 = rx < rnw ? s_rc[li][rx] : 0u;
-                    const uint32_t r1 = rx + 1 < rnw ? s_rc[li][rx + 1] : 0u;
-                    const uint32_t r2 = rx + 2 < rnw ? s_rc[li][rx + 2] : 0u;
-                    const uint32_t g0 = gx < gnw ? s_gc[li][gx] : 0u;
-                    const uint32_t g1 = gx + 1 < gnw ? s_gc[li][gx + 1] : 0u;
-                    const uint32_t g2 = gx + 2 < gnw ? s_gc[li][gx + 2] : 0u;
+                    const uint32_t c0 = rx < rnw ? src_[rx] : 0u;
+                    const uint32_t c1 = rx + 1 < rnw ? src_[rx + 1] : 0u;
+                    const uint32_t c2 = rx + 2 < rnw ? src_[rx + 2] : 0u;
+                    const uint32_t g0 = gx < gnw ? sgc[gx] : 0u;
+                    const uint32_t g1 = gx + 1 < gnw ? sgc[gx + 1] : 0u;
+                    const uint32_t g2 = gx + 2 < gnw ? sgc[gx + 2] : 0u;
                     int rk = ri & 15, gk = gi & 15;
                     for (int i = 0; i < cnt; ++i) {
                         const int p = p0 + i;
                         const uint32_t op = bits2(o0, o1, o2, n - 1 - p - 16 * jb);
                         if (p >= lead) {
-                            orow[p - lead] = op != SRM_OP_TOP ? base_char(bits2(r0, r1, r2, rk)) : '_';
+                            orow[p - lead] = op != SRM_OP_TOP ? base_char(bits2(c0, c1, c2, rk)) : '_';
                             grow[p - lead] = op != SRM_OP_LEFT ? base_char(bits2(g0, g1, g2, gk)) : '_';
                         }
                         if (op != SRM_OP_LEFT) ++gk;
@@ -656,27 +617,30 @@ __global__ void __launch_bounds__(GB) rec_write_group(RecArgs a)
                 if (u <= ndsc + 1)
                     tail[u] = u == 0 ? '\t' : (u == ndsc + 1 ? '\n' : digit_at(score, ndsc, u - 1));
             }
-            __syncthreads();
-            // ---- copy the run's text out
-            {
-                const uint32_t *ml = s_meta[nl - 1];
-                const uint64_t end_off = ((uint64_t)ml[M_OFFLO] | ((uint64_t)ml[M_OFFHI] << 32)) + ml[M_LEN];
-                const int end = al0 + (int)(end_off - off0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // ---- copy the line out
+            if (act) {
+                uint32_t *o32 = (uint32_t *)(a.out + (loff - (uint64_t)al));
+                const int end = al + (int)llen;
                 const int nd = (end + 3) >> 2;
-                uint32_t *o32 = (uint32_t *)(a.out + (off0 - (uint64_t)al0));
-                for (int i = tid; i < nd; i += GB) {
-                    const uint32_t v = s_text[i];
+                const uint32_t *t32 = s_text[wv][q];
+                for (int i = u; i < nd; i += TPL) {
+                    const uint32_t v = t32[i];
                     const int b0 = 4 * i;
-                    if (b0 >= al0 && b0 + 4 <= end) {
+                    if (b0 >= al && b0 + 4 <= end) {
                         o32[i] = v;
                     } else {
                         char *o8 = (char *)(o32 + i);
                         for (int b = 0; b < 4; ++b)
-                            if (b0 + b >= al0 && b0 + b < end) o8[b] = (char)(v >> (8 * b));
+                            if (b0 + b >= al && b0 + b < end) o8[b] = (char)(v >> (8 * b));
                     }
                 }
             }
-            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }
@@ -693,14 +657,13 @@ hipError_t launch_rec_measure(const RecArgs &a, hipStream_t st)
 hipError_t launch_rec_write(const RecArgs &a, hipStream_t st)
 {
     if (a.n <= 0) return hipSuccess;
-    // runs of short lines: a block per tile of 256 candidates, a grid that strides
-    const int gtiles = (a.n + GB - 1) / GB;
-    hipLaunchKernelGGL(rec_write_group, dim3(gtiles < 4096 ? gtiles : 4096), dim3(GB), 0, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // the other lines: a wave per tile of 64 candidates
+    // a wave per tile of 64 candidates for both kernels: the short lines four at a
+    // time, then the others one at a time
     const int tiles = (a.n + 63) / 64;
     const int waves = tiles < 65536 * 4 ? tiles : 65536 * 4;
+    hipLaunchKernelGGL(rec_write_quad, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rec_write, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
     return hipGetLastError();
 }
