@@ -182,20 +182,55 @@ struct Store {
     }
 };
 
-// does the ascending run flat[lo, min(hi, end)) -- cut at its first 0, as the
-// reference's 0-terminated scans are -- hold the position tg?
-SRM_HD inline bool run_has(const uint32_t *flat, uint64_t lo, uint64_t hi, uint64_t end, long tg)
+// first index of the ascending run flat[lo, hi) whose value is >= t
+SRM_HD inline uint64_t run_lower(const uint32_t *flat, uint64_t lo, uint64_t hi, long t)
+{
+    if (t <= 0) return lo;
+    if (t > 0xFFFFFFFFl) return hi;
+    const uint32_t u = (uint32_t)t;
+    while (lo < hi) {
+        const uint64_t m = (lo + hi) >> 1;
+        if (flat[m] < u) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// The pair passes' matches of one list in the reference's scan order.  The scan
+// compares every target tg(err), err = -lim .. lim, with the ascending run
+// flat[lo, min(hi, end)) -- cut at its first 0, as the reference's 0-terminated
+// scans are -- and, in the first pass, first with one more value p0 (p0 == 0:
+// none).  The targets are consecutive (ascending from t_lo when up, descending
+// from t_hi otherwise), so the matches are the run's values in [t_lo, t_hi] found
+// by one search and a walk, p0 merged in before a run value of the same target.
+// Counts them into *hits and returns true when the count reaches need.
+SRM_HD inline bool scan_run(const uint32_t *flat, uint64_t lo, uint64_t hi, uint64_t end,
+                            long t_lo, long t_hi, bool up, uint32_t p0, int need, int &hits)
 {
     if (hi > end) hi = end;
-    if (lo >= hi || tg <= 0 || tg > 0xFFFFFFFFl || flat[lo] == 0) return false;
-    const uint32_t t = (uint32_t)tg;
-    uint64_t l = lo, h = hi;
-    while (l < h) {
-        const uint64_t m = (l + h) >> 1;
-        if (flat[m] < t) l = m + 1;
-        else h = m;
+    if (lo >= hi || flat[lo] == 0) hi = lo;    // an empty run
+    bool p0_left = p0 != 0 && (long)p0 >= t_lo && (long)p0 <= t_hi;
+    if (up) {
+        uint64_t i = run_lower(flat, lo, hi, t_lo);
+        for (;;) {
+            const bool run_next = i < hi && (long)flat[i] <= t_hi;
+            if (p0_left && (!run_next || (long)p0 <= (long)flat[i])) {
+                p0_left = false;
+                if (++hits == need) return true;
+                continue;
+            }
+            if (!run_next) return false;
+            if (++hits == need) return true;
+            ++i;
+        }
     }
-    return l < hi && flat[l] == t;
+    // descending (second pass only: no p0)
+    uint64_t i = run_lower(flat, lo, hi, t_hi + 1);
+    while (i > lo && (long)flat[i - 1] >= t_lo) {
+        if (++hits == need) return true;
+        --i;
+    }
+    return false;
 }
 
 // add_start_position, hash.c:40-178: 0 first entry, 1 rejected, 2 added or
@@ -258,13 +293,21 @@ SRM_HD inline void seashore(const FArgs &a, uint32_t r, Store &s)
                     }
                     if (s.ovf) return;
                 }
-                int possible = 1;
+                // the pair passes, hash.c:606-794.  Every match raises `possible` and
+                // calls add_start_position, which acts only when possible equals
+                // exact_tresh (Q13): the need-th match of the two passes in scan order,
+                // need = exact_tresh - 1, is the only one that can insert; the scan stops
+                // there.
+                const int need = P.exact - 1;
+                int hits = 0;
+                bool found = need < 1;
+                int32_t hpos = 0;
+                int hflag = 0;
                 // first pass, hash.c:606-692.  For every err the reference scans
                 // D(b, off), then D(b, 1), D(b, 2), ... up to the next 0 (Q12): the
                 // element at a's index, then one ascending run of list b (or of the
-                // list after b when b is empty).  A run holds each position once, so
-                // a binary search per err finds what the scan finds, in scan order.
-                for (int b = aa + 1; b < half; ++b) {
+                // list after b when b is empty).
+                for (int b = aa + 1; b < half && !found; ++b) {
                     const uint32_t P0 = D(b, off);
                     if (P0 == 0) continue;                 // the scan never starts
                     const long rstart = (long)b * q + 1;
@@ -275,65 +318,53 @@ SRM_HD inline void seashore(const FArgs &a, uint32_t r, Store &s)
                     } else if ((uint64_t)lb + b + 2 <= a.list_base[a.n_reads]) {
                         hi = (uint64_t)a.list_start[lb + b + 2] - 1;   // the next list
                     }
-                    int flag;
-                    long pos;
-                    if (b == half - 1 && fqm) {
-                        flag = 1 + dr;
-                        pos = gs + 1;
-                    } else if (aa == half - 1) {
-                        flag = ((long)Pa + q > L ? 3 : 2) + dr;
-                        pos = gs - k + 1;
-                    } else if (fqm) {
-                        flag = 1 + dr;
-                        pos = gs + 1;
-                    } else {
-                        flag = 3 + dr;
-                        pos = gs - k + 1;
-                    }
-                    for (int err = -k; err <= k; ++err) {
-                        const long tg = gs + rstart + err;
-                        if ((long)P0 == tg) {
-                            ++possible;
-                            add_start(s, (int32_t)pos, flag, possible, P.exact, k);
-                            if (s.ovf) return;
-                        }
-                        if (run_has(a.flat, lo, hi, flat_end, tg)) {
-                            ++possible;
-                            add_start(s, (int32_t)pos, flag, possible, P.exact, k);
-                            if (s.ovf) return;
+                    if (scan_run(a.flat, lo, hi, flat_end, gs + rstart - k, gs + rstart + k, true,
+                                 P0, need, hits)) {
+                        found = true;
+                        if (b == half - 1 && fqm) {
+                            hflag = 1 + dr;
+                            hpos = (int32_t)(gs + 1);
+                        } else if (aa == half - 1) {
+                            hflag = ((long)Pa + q > L ? 3 : 2) + dr;
+                            hpos = (int32_t)(gs - k + 1);
+                        } else if (fqm) {
+                            hflag = 1 + dr;
+                            hpos = (int32_t)(gs + 1);
+                        } else {
+                            hflag = 3 + dr;
+                            hpos = (int32_t)(gs - k + 1);
                         }
                     }
                 }
                 // second pass, hash.c:695-794: each err scans list half + j from its
-                // start up to its 0
-                for (int j = 0; j < half; ++j) {
+                // start up to its 0; the targets gs + nin + err run downwards when
+                // j < a (the reference's err is negated there)
+                for (int j = 0; j < half && !found; ++j) {
                     const int lim = half - 1 - j - aa < 0 ? -(half - 1 - j - aa) : half - 1 - j - aa;
                     const long nin = (long)(half - j - 1) * q + 1 + rest;
                     const uint64_t lo = a.list_start[lb + half + j];
                     const uint64_t hi = (uint64_t)a.list_start[lb + half + j + 1] - 1;
-                    int flag;
-                    long pos;
-                    if (j == 0 && fqm) {
-                        flag = 1 + dr;
-                        pos = gs + 1;
-                    } else if (j == 0) {
-                        flag = (nin + q > L ? 3 : 2) + dr;
-                        pos = gs - k + 1;
-                    } else if (fqm) {
-                        flag = 1 + dr;
-                        pos = gs + 1;
-                    } else {
-                        flag = 3 + dr;
-                        pos = gs - k + 1;
-                    }
-                    for (int err = -lim; err <= lim; ++err) {
-                        const int aerr = (j - aa < 0) ? -err : err;
-                        if (run_has(a.flat, lo, hi, flat_end, gs + nin + aerr)) {
-                            ++possible;
-                            add_start(s, (int32_t)pos, flag, possible, P.exact, k);
-                            if (s.ovf) return;
+                    if (scan_run(a.flat, lo, hi, flat_end, gs + nin - lim, gs + nin + lim, j - aa >= 0,
+                                 0u, need, hits)) {
+                        found = true;
+                        if (j == 0 && fqm) {
+                            hflag = 1 + dr;
+                            hpos = (int32_t)(gs + 1);
+                        } else if (j == 0) {
+                            hflag = (nin + q > L ? 3 : 2) + dr;
+                            hpos = (int32_t)(gs - k + 1);
+                        } else if (fqm) {
+                            hflag = 1 + dr;
+                            hpos = (int32_t)(gs + 1);
+                        } else {
+                            hflag = 3 + dr;
+                            hpos = (int32_t)(gs - k + 1);
                         }
                     }
+                }
+                if (found && need >= 1) {
+                    add_start(s, hpos, hflag, P.exact, P.exact, k);
+                    if (s.ovf) return;
                 }
             }
             ++off;
