@@ -109,16 +109,32 @@ __global__ void __launch_bounds__(FB) fl_pair(FArgs a)
     }
     if (active) seashore(a, a.todo ? a.todo[j] : j, s);
     const uint32_t ne = (active && !s.ovf) ? s.n : 0, np = (active && !s.ovf) ? s.np : 0;
-    // one atomic per wave reserves the wave's output
+    // one atomic pair per block reserves the block's output (the two counters are
+    // words that every wave would otherwise update); the waves take consecutive
+    // parts of it
+    __shared__ uint32_t s_sum[FB / 64][2], s_base[2];
     const uint32_t ie = wave_incl_sum(ne), ip = wave_incl_sum(np);
-    const int lane = threadIdx.x & 63;
-    uint32_t be = 0, bp = 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 63) {
-        be = atomicAdd(&a.counters[0], ie);
-        bp = atomicAdd(&a.counters[1], ip);
+        s_sum[wv][0] = ie;
+        s_sum[wv][1] = ip;
     }
-    be = __shfl(be, 63, 64) + ie - ne;
-    bp = __shfl(bp, 63, 64) + ip - np;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t te = 0, tp = 0;
+        for (int v = 0; v < FB / 64; ++v) {
+            te += s_sum[v][0];
+            tp += s_sum[v][1];
+        }
+        s_base[0] = te ? atomicAdd(&a.counters[0], te) : 0u;
+        s_base[1] = tp ? atomicAdd(&a.counters[1], tp) : 0u;
+    }
+    __syncthreads();
+    uint32_t be = s_base[0] + ie - ne, bp = s_base[1] + ip - np;
+    for (int v = 0; v < wv; ++v) {
+        be += s_sum[v][0];
+        bp += s_sum[v][1];
+    }
     if (!active) return;
     for (uint32_t e = 0; e < ne; ++e) {
         a.out_pos[be + e] = s.P(e);

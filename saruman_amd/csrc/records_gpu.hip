@@ -15,6 +15,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "nw_device.h"
+#include "rec_ops.h"
 
 namespace srm {
 
@@ -22,41 +23,13 @@ namespace {
 
 constexpr int RB = 256;
 
-__device__ __forceinline__ uint32_t rcode(const uint32_t *w, int64_t idx)
-{
-    return (w[idx >> 4] >> ((uint32_t)(idx & 15) * 2u)) & 3u;
-}
-
-__device__ __forceinline__ uint32_t op_at(const uint32_t *ops, int j)
-{
-    return (ops[j >> 4] >> ((j & 15) * 2)) & 3u;
-}
-
 // characters %d prints for v
 __device__ __forceinline__ int ndig(int v)
 {
-    uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
-    int n = v < 0 ? 2 : 1;
-    while (u >= 10) {
-        u /= 10;
-        ++n;
-    }
-    return n;
-}
-
-// writes v as %d does; returns the end
-__device__ __forceinline__ char *put_int(char *o, int v)
-{
-    const int nd = ndig(v);
-    char *e = o + nd;
-    uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
-    char *q = e;
-    do {
-        *--q = (char)('0' + u % 10);
-        u /= 10;
-    } while (u);
-    if (v < 0) *o = '-';
-    return e;
+    const uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+    return 1 + (v < 0) + (u >= 10u) + (u >= 100u) + (u >= 1000u) + (u >= 10000u) +
+           (u >= 100000u) + (u >= 1000000u) + (u >= 10000000u) + (u >= 100000000u) +
+           (u >= 1000000000u);
 }
 
 __device__ __forceinline__ int name_len(const RecArgs &a, int32_t id)
@@ -70,55 +43,16 @@ __device__ __forceinline__ int name_len(const RecArgs &a, int32_t id)
     return a.prefix_len + ndig(id);
 }
 
-// a transcript read op by op, one word load per 16 ops
-struct OpCursor {
-    const uint32_t *ops;
-    int wi = -1;
-    uint32_t word = 0;
-    __device__ uint32_t at(int j)
-    {
-        const int w = j >> 4;
-        if (w != wi) {
-            wi = w;
-            word = ops[w];
-        }
-        return (word >> ((j & 15) * 2)) & 3u;
-    }
-};
-
 // leading gaps, closing gaps, indel scan and printed length of one hit
-// (cuda-functions.cu:866-980); returns the line length
+// (cuda-functions.cu:866-980), a transcript word at a time; returns the line length
 __device__ uint64_t measure_one(const RecArgs &a, int t)
 {
     const srm_cand c = a.cand[t];
     const srm_result r = a.res[t];
-    const uint32_t *ops = a.ops + c.ops_off;
     const int n = r.n_ops, w = c.w, L = c.L;
-    // slot position p holds transcript op n-1-p (the reference reverses the rows);
-    // the forward scans and the closing-gap scan each keep their current word
-    OpCursor fw{ops}, bw{ops};
-    auto R_gap = [&](int p) { return fw.at(n - 1 - p) == SRM_OP_TOP; };
-    auto G_gap = [&](int p) { return fw.at(n - 1 - p) == SRM_OP_LEFT; };
-    int lead = 0;
-    while (lead < n && R_gap(lead)) ++lead;                      // :878-886
-    int trail = 0;
-    for (int p = n - 1; p >= 0 && bw.at(n - 1 - p) == SRM_OP_TOP; --p) ++trail;   // :897-907
-    int del = 0, ins = 0, i = 0, z = 0, eor = 0;                 // :910-959
-    while (i < w - trail - lead + z) {
-        const int p = lead + i;
-        if (p >= n) {          // NUL: the zero padding after the emitted row
-            eor = i;
-            break;
-        }
-        if (R_gap(p)) {
-            ++del;
-            ++z;
-        } else if (G_gap(p)) {
-            ++ins;
-            ++z;
-        }
-        ++i;
-    }
+    const uint32_t *ops = a.ops + c.ops_off;
+    const srm_rec::Measure m = srm_rec::measure_ops_by([ops](int W) { return ops[W]; }, n, w);
+    const int lead = m.lead, del = m.del, ins = m.ins, eor = m.eor, trail = m.trail;
     const short chars = eor > 0 ? (short)(eor - trail) : (short)(L + del);   // :972-980 (Q8)
     const int cap = w + (int)c.k;
     const int lim = chars < 0 ? cap : chars;
@@ -132,22 +66,39 @@ __device__ uint64_t measure_one(const RecArgs &a, int t)
                       ndig((int)r.score) + 7);
 }
 
+// a grid that strides over the candidates; the hits are summed per block and
+// added to the counter with one atomic per block (the counter is one word that
+// every block would otherwise update once per wave)
 __global__ void __launch_bounds__(RB) rec_measure(RecArgs a)
 {
-    const int t = blockIdx.x * RB + threadIdx.x;
-    const bool hit = t < a.n && a.res[t].score <= (int)a.cand[t].k;
-    if (t < a.n) a.len[t] = hit ? measure_one(a, t) : 0;
-    if (t == a.n) a.len[t] = 0;                     // the scan's tail element
-    const uint64_t b = __ballot(hit);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(a.hits, (uint32_t)__popcll(b));
+    __shared__ uint32_t s_hits[RB / 64];
+    uint32_t hits = 0;
+    for (int t = blockIdx.x * RB + threadIdx.x; t <= a.n; t += gridDim.x * RB) {
+        if (t == a.n) {
+            a.len[t] = 0;                           // the scan's tail element
+            continue;
+        }
+        const bool hit = a.res[t].score <= (int)a.cand[t].k;
+        a.len[t] = hit ? measure_one(a, t) : 0;
+        hits += hit;
+    }
+    // block sum: waves by reduction in registers, then the block's waves in LDS
+    for (int d = 32; d >= 1; d >>= 1) hits += (uint32_t)__shfl_xor((int)hits, d, 64);
+    if ((threadIdx.x & 63) == 0) s_hits[threadIdx.x >> 6] = hits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = 0;
+        for (int w = 0; w < RB / 64; ++w) b += s_hits[w];
+        if (b) atomicAdd(a.hits, b);
+    }
 }
 
-// line sizes of rec_write_quad (below), which writes the lines grouped_line() accepts
-constexpr int QL = 4;                // lines per wave
-constexpr int TPL = 16;              // lanes per line
-constexpr int CHUNK = 32;            // printed positions per lane at most
-constexpr int MAXP = TPL * CHUNK;    // printed positions per line (512)
-constexpr int LCAP = 1024;           // text bytes per line
+// line sizes of rec_write_lines (below), which writes the lines grouped_line() accepts
+constexpr int QL = 8;                // lines per wave
+constexpr int TPL = 8;               // lanes per line
+constexpr int CHUNK = 40;            // printed positions per lane at most
+constexpr int MAXP = TPL * CHUNK;    // printed positions per line (320)
+constexpr int LCAP = 704;            // text bytes per line
 constexpr int WPL = MAXP / 16 + 3;   // staged words per line: transcript, read codes, window codes
 constexpr int NAMEW = 64;            // staged name words per line (name_stride <= 256)
 
@@ -237,7 +188,7 @@ __global__ void __launch_bounds__(RB) rec_write(RecArgs a)
             rw_l[0] = rp[0];
             rw_l[1] = rp[1];
         }
-        // the lines rec_write_quad leaves: long ones and raw-byte candidates
+        // the lines rec_write_lines leaves: long ones and raw-byte candidates
         uint64_t hits = __ballot(len_l != 0 && !grouped_line(len_l, inf_l.z, cw_l[4] >> 16));
         if (!hits) continue;
         const uint64_t off_l = len_l ? a.off[tl] : 0;
@@ -393,28 +344,23 @@ __global__ void __launch_bounds__(RB) rec_write(RecArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// rec_write_quad: four lines per wave at a time, TPL = 16 lanes per line.  A wave
-// takes 64 consecutive candidates (their length, offset and metadata in one
+// rec_write_lines: QL = 8 lines per wave at a time, TPL = 8 lanes per line.  A
+// wave takes 64 consecutive candidates (their length, offset and metadata in one
 // coalesced load per field, a candidate per lane) and writes the lines
-// grouped_line() accepts, four at a time:
+// grouped_line() accepts, eight at a time:
 //   stage   each line's lanes load the transcript words and the read and window
 //           code words its rows need (and a name from the table) into the line's
-//           LDS slot: the four lines' loads are in flight together
-//   build   header, rows and tail into the slot at the output's byte alignment; the
-//           rows split evenly over the line's lanes (at most CHUNK printed positions
-//           each), placed by a scan of the runs' row and column advances
+//           LDS slot: the eight lines' loads are in flight together
+//   build   header and tail (the four numbers a lane each), then the rows into
+//           the slot at the output's byte alignment: the positions split over the
+//           line's lanes in runs of a multiple of 4 (at most CHUNK each), placed by
+//           a scan of the runs' row and column advances, four characters per row
+//           at a time
 //   copy    the line's lanes store it with dwords, bytes only in the first and last
 //           dwords, which the neighbouring lines share
 // Every step of a line stays within its wave: no block barrier.  Long lines and
 // raw-byte candidates are rec_write's.
 constexpr int SLOT_TEXT = LCAP / 4 + 1;    // text words per slot (a line at its alignment)
-
-// 2-bit field idx (0..47) of three consecutive words
-__device__ __forceinline__ uint32_t bits2(uint32_t w0, uint32_t w1, uint32_t w2, int idx)
-{
-    const uint32_t w = idx < 16 ? w0 : (idx < 32 ? w1 : w2);
-    return (w >> ((idx & 15) * 2)) & 3u;
-}
 
 __device__ __forceinline__ uint32_t from_lane(uint32_t v, int src)
 {
@@ -428,7 +374,28 @@ __device__ __forceinline__ uint64_t bitrange64(int lo, int hi)   // bits [lo, hi
     return top & ~((1ull << lo) - 1ull);
 }
 
-__global__ void __launch_bounds__(RB) rec_write_quad(RecArgs a)
+// four 2-bit fields from field f of a staged word array (word f >> 4 and the next)
+__device__ __forceinline__ uint32_t fields4(const uint32_t *w, int f)
+{
+    const int x = f >> 4;
+    const uint64_t v = (uint64_t)w[x] | ((uint64_t)w[x + 1] << 32);
+    return (uint32_t)(v >> (2 * (f & 15))) & 0xFFu;
+}
+
+// v as %d prints it, nd = ndig(v) characters at o
+__device__ __forceinline__ void put_dec(char *o, int32_t v, int nd)
+{
+    uint32_t x = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+    int e = nd;
+    do {
+        const uint32_t d = x / 10u;
+        o[--e] = (char)('0' + (x - 10u * d));
+        x = d;
+    } while (x);
+    if (v < 0) o[0] = '-';
+}
+
+__global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
 {
     __shared__ uint32_t s_text[RB / 64][QL][SLOT_TEXT];
     __shared__ uint32_t s_ops[RB / 64][QL][WPL], s_rc[RB / 64][QL][WPL], s_gc[RB / 64][QL][WPL];
@@ -436,6 +403,7 @@ __global__ void __launch_bounds__(RB) rec_write_quad(RecArgs a)
     __shared__ char s_prefix[256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int q = lane / TPL, u = lane % TPL;      // this lane's line slot and place in it
+    const int sb = lane & ~(TPL - 1);              // the slot's first lane
     if (!a.names)
         for (int i = tid; i < a.prefix_len; i += RB) s_prefix[i] = a.prefix[i];
     __syncthreads();
@@ -519,63 +487,73 @@ __global__ void __launch_bounds__(RB) rec_write_quad(RecArgs a)
             const int al = (int)(loff & 3u);
             char *line = (char *)s_text[wv][q] + al;
             if (act) {
-                const int nds = ndig(start), ndt = ndig(stop), ndi = ndig(lid), ndsc = ndig(score);
+                // the line's four numbers, a lane each: start, stop, the read id (when
+                // names are prefix + id) and the score; their widths to every lane
+                const int32_t num = u == 0 ? start : u == 1 ? stop : u == 2 ? lid : score;
+                const int ndn = ndig(num);
+                const int nds = __shfl(ndn, sb, 64), ndt = __shfl(ndn, sb + 1, 64);
+                const int ndi = __shfl(ndn, sb + 2, 64), ndsc = __shfl(ndn, sb + 3, 64);
+                const char *sname = (const char *)s_name[wv][q];
                 int name_n;
                 if (a.names) {
-                    const char *sn = (const char *)s_name[wv][q];
                     name_n = 0;
-                    while (name_n < a.name_stride - 1 && sn[name_n]) ++name_n;
+                    while (name_n < a.name_stride - 1 && sname[name_n]) ++name_n;
                 } else {
                     name_n = a.prefix_len + ndi;
                 }
                 const int hdr = name_n + 1 + nds + 1 + ndt + 1 + 2 + 1;
+                // header: name \t start \t stop \t << or >> \t, every character but
+                // the digits (x counts from the end of the name)
                 for (int h = u; h < hdr; h += TPL) {
-                    char ch;
-                    int x = h;
-                    if (x < name_n) {
-                        if (a.names)
-                            ch = ((const char *)s_name[wv][q])[x];
-                        else
-                            ch = x < a.prefix_len ? s_prefix[x] : digit_at(lid, ndi, x - a.prefix_len);
-                    } else if ((x -= name_n) == 0) {
-                        ch = '\t';
-                    } else if ((x -= 1) < nds) {
-                        ch = digit_at(start, nds, x);
-                    } else if ((x -= nds) == 0) {
-                        ch = '\t';
-                    } else if ((x -= 1) < ndt) {
-                        ch = digit_at(stop, ndt, x);
-                    } else if ((x -= ndt) == 0 || x == 3) {
-                        ch = '\t';
-                    } else {
-                        ch = flag > 3 ? '<' : '>';
+                    const int x = h - name_n;
+                    if (x < 0) {
+                        if (a.names) line[h] = sname[h];
+                        else if (h < a.prefix_len) line[h] = s_prefix[h];
+                    } else if (x == 0 || x == 1 + nds || x == 2 + nds + ndt || x == 5 + nds + ndt) {
+                        line[h] = '\t';
+                    } else if (x == 3 + nds + ndt || x == 4 + nds + ndt) {
+                        line[h] = flag > 3 ? '<' : '>';
                     }
-                    line[h] = ch;
+                }
+                // the digits: each of the four lanes prints its number as %d does
+                {
+                    int at = -1;
+                    if (u == 0) at = name_n + 1;
+                    else if (u == 1) at = name_n + 2 + nds;
+                    else if (u == 2 && !a.names) at = a.prefix_len;
+                    else if (u == 3) at = hdr + 2 * rn + 2;
+                    if (at >= 0) put_dec(line + at, num, ndn);
+                }
+                if (u == 4) {
+                    line[hdr + rn] = '\t';                    // between the rows
+                    line[hdr + 2 * rn + 1] = '\t';            // before the score
+                    line[hdr + 2 * rn + 2 + ndsc] = '\n';
                 }
                 char *orow = line + hdr;
                 char *grow = orow + rn + 1;
-                // the line's positions split evenly over its lanes (at most CHUNK each)
-                const int chunk = (endp + TPL - 1) / TPL;
+                // the line's positions split over its lanes in runs of a multiple of 4
+                // (at most CHUNK each)
+                const int chunk = (((endp + TPL - 1) / TPL) + 3) & ~3;
                 const int p0 = u * chunk;
                 const int cnt = endp - p0 < 0 ? 0 : (endp - p0 > chunk ? chunk : endp - p0);
-                uint32_t o0 = 0, o1 = 0, o2 = 0;
-                int jb = 0;
                 uint32_t drow = 0, dcol = 0;
                 if (cnt > 0) {
-                    const int jm = n - 1 - p0 - (cnt - 1);
-                    jb = jm >> 4;
-                    const int x = jb - wlo;
-                    o0 = sops[x];
-                    o1 = x + 1 < nw_ops ? sops[x + 1] : 0u;
-                    o2 = x + 2 < nw_ops ? sops[x + 2] : 0u;
                     // advances of the run: ops other than LEFT move the row, ops other
-                    // than TOP the column, counted over its 2-bit fields [jm, jm + cnt)
-                    const int b0 = 2 * (jm - 16 * jb), b1 = b0 + 2 * cnt;
+                    // than TOP the column, counted over its ops [jm, jm + cnt), at most
+                    // 40 fields from field jm & 15 of staged word (jm >> 4) - wlo
+                    const int jm = n - 1 - p0 - (cnt - 1);
+                    const int x = (jm >> 4) - wlo;
+                    const uint32_t o0 = sops[x];
+                    const uint32_t o1 = x + 1 < nw_ops ? sops[x + 1] : 0u;
+                    const uint32_t o2 = x + 2 < nw_ops ? sops[x + 2] : 0u;
+                    const uint32_t o3 = x + 3 < nw_ops ? sops[x + 3] : 0u;
+                    const int b0 = 2 * (jm & 15), b1 = b0 + 2 * cnt;
                     const uint64_t X = (uint64_t)o0 | ((uint64_t)o1 << 32);
+                    const uint64_t Y = (uint64_t)o2 | ((uint64_t)o3 << 32);
                     const uint64_t mx = bitrange64(b0 < 64 ? b0 : 64, b1 < 64 ? b1 : 64);
                     const uint64_t my = bitrange64(b0 > 64 ? b0 - 64 : 0, b1 > 64 ? b1 - 64 : 0);
                     const uint64_t xl = X & e64, xh = (X >> 1) & e64;
-                    const uint64_t yl = (uint64_t)o2 & e64, yh = ((uint64_t)o2 >> 1) & e64;
+                    const uint64_t yl = Y & e64, yh = (Y >> 1) & e64;
                     const int tops = __popcll(xl & ~xh & mx) + __popcll(yl & ~yh & my);
                     const int lefts = __popcll(xh & ~xl & mx) + __popcll(yh & ~yl & my);
                     drow = (uint32_t)(cnt - lefts);
@@ -591,31 +569,45 @@ __global__ void __launch_bounds__(RB) rec_write_quad(RecArgs a)
                 }
                 const int row = end_row + (int)(sr - drow), col = end_col + (int)(sc - dcol);
                 if (cnt > 0) {
-                    const int ri = (int)(read_off + (uint32_t)col) - 16 * rlo;
-                    const int gi = (int)(win_off + (uint32_t)row) - 16 * glo;
-                    const int rx = ri >> 4, gx = gi >> 4;
-                    const uint32_t c0 = rx < rnw ? src_[rx] : 0u;
-                    const uint32_t c1 = rx + 1 < rnw ? src_[rx + 1] : 0u;
-                    const uint32_t c2 = rx + 2 < rnw ? src_[rx + 2] : 0u;
-                    const uint32_t g0 = gx < gnw ? sgc[gx] : 0u;
-                    const uint32_t g1 = gx + 1 < gnw ? sgc[gx + 1] : 0u;
-                    const uint32_t g2 = gx + 2 < gnw ? sgc[gx + 2] : 0u;
-                    int rk = ri & 15, gk = gi & 15;
-                    for (int i = 0; i < cnt; ++i) {
-                        const int p = p0 + i;
-                        const uint32_t op = bits2(o0, o1, o2, n - 1 - p - 16 * jb);
-                        if (p >= lead) {
-                            orow[p - lead] = op != SRM_OP_TOP ? base_char(bits2(c0, c1, c2, rk)) : '_';
-                            grow[p - lead] = op != SRM_OP_LEFT ? base_char(bits2(g0, g1, g2, gk)) : '_';
+                    // code fields of the run's first cell in the staged read and window words
+                    int rf = (int)(read_off + (uint32_t)col) - 16 * rlo;
+                    int gf = (int)(win_off + (uint32_t)row) - 16 * glo;
+                    const int jlo = 16 * wlo;
+                    // four positions at a time: their ops, then four characters per row
+                    // from one byte permute each ('_' for a gap)
+                    for (int i = 0; i < cnt; i += 4) {
+                        // ops of positions p0 + i .. p0 + i + 3: fields j .. j + 3 with
+                        // j = n - 4 - p0 - i (descending positions); fields below the
+                        // staged words belong to positions past the line (taken as 0)
+                        const int j = n - 4 - p0 - i;
+                        const uint32_t og = j >= jlo ? fields4(sops, j - jlo)
+                                                     : fields4(sops, 0) << (2 * (jlo - j));
+                        const uint32_t rc8 = fields4(src_, rf), gc8 = fields4(sgc, gf);
+                        uint32_t rsel = 0, gsel = 0;
+                        int ro = 0, go = 0;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const uint32_t op = (og >> (6 - 2 * t)) & 3u;   // position p0 + i + t
+                            const bool top = op == SRM_OP_TOP, left = op == SRM_OP_LEFT;
+                            rsel |= (top ? 4u : (rc8 >> (2 * ro)) & 3u) << (8 * t);
+                            gsel |= (left ? 4u : (gc8 >> (2 * go)) & 3u) << (8 * t);
+                            ro += top ? 0 : 1;
+                            go += left ? 0 : 1;
                         }
-                        if (op != SRM_OP_LEFT) ++gk;
-                        if (op != SRM_OP_TOP) ++rk;
+                        const uint32_t rch = __builtin_amdgcn_perm(0x5F5F5F5Fu, 0x54474341u, rsel);
+                        const uint32_t gch = __builtin_amdgcn_perm(0x5F5F5F5Fu, 0x54474341u, gsel);
+                        const int pl = p0 + i - lead;          // row place of position p0 + i
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            if (i + t < cnt && pl + t >= 0) {
+                                orow[pl + t] = (char)(rch >> (8 * t));
+                                grow[pl + t] = (char)(gch >> (8 * t));
+                            }
+                        }
+                        rf += ro;
+                        gf += go;
                     }
                 }
-                if (u == 0) orow[rn] = '\t';
-                char *tail = grow + rn;
-                if (u <= ndsc + 1)
-                    tail[u] = u == 0 ? '\t' : (u == ndsc + 1 ? '\n' : digit_at(score, ndsc, u - 1));
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -650,18 +642,19 @@ __global__ void __launch_bounds__(RB) rec_write_quad(RecArgs a)
 hipError_t launch_rec_measure(const RecArgs &a, hipStream_t st)
 {
     if (a.n < 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(rec_measure, dim3(a.n / RB + 1), dim3(RB), 0, st, a);
+    const int blocks = a.n / RB + 1;
+    hipLaunchKernelGGL(rec_measure, dim3(blocks < 2048 ? blocks : 2048), dim3(RB), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_rec_write(const RecArgs &a, hipStream_t st)
 {
     if (a.n <= 0) return hipSuccess;
-    // a wave per tile of 64 candidates for both kernels: the short lines four at a
-    // time, then the others one at a time
+    // a wave per tile of 64 candidates for both kernels: the short lines eight at
+    // a time, then the others one at a time
     const int tiles = (a.n + 63) / 64;
     const int waves = tiles < 65536 * 4 ? tiles : 65536 * 4;
-    hipLaunchKernelGGL(rec_write_quad, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
+    hipLaunchKernelGGL(rec_write_lines, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rec_write, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
