@@ -62,12 +62,15 @@ def kernel_sha() -> str:
     return h.hexdigest()[:12]
 
 
+TUNING_KNOBS = ("merge_below", "class_streams", "tile_cands", "block_waves", "split_len")
+
+
 def tuning_key(tuning):
-    """The launch tuning as reported: "defaults" or {blocks_per_cu, merge_below}."""
-    if not tuning or (not any(tuning["blocks_per_cu"]) and tuning.get("merge_below", 0) <= 1
-                      and tuning.get("class_streams", 0) == 0):
+    """The launch tuning as reported: "defaults" or the knobs that differ."""
+    if not tuning or (not any(tuning.get("blocks_per_cu", [0])) and tuning.get("merge_below", 0) <= 1
+                      and not any(tuning.get(k, 0) for k in TUNING_KNOBS[1:])):
         return "defaults"
-    return {k: tuning.get(k, 0) for k in ("blocks_per_cu", "merge_below", "class_streams")}
+    return {k: tuning.get(k, 0) for k in ("blocks_per_cu",) + TUNING_KNOBS}
 
 
 def profiled_traffic(workload, candidates, tuning):

@@ -231,17 +231,25 @@ int srm_ctx_set_genome(srm_ctx *ctx, const uint32_t *words, size_t n_words);
 int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src);
 
 /*
- * Launch shape of the banded kernel, chosen by the autotuner
- * (saruman_amd/tune.py) and applied to jobs uploaded afterwards.
+ * Launch shape of the banded kernels, chosen by the autotuner
+ * (saruman_amd/tune.py) and applied to jobs uploaded afterwards.  Every field 0
+ * is the default launch.
  * blocks_per_cu[nw]: persistent 256-thread blocks per CU for band class nw
- * (8*nw band cells per lane), 0 = as many as stay resident; merge_below: band
- * classes narrower than this run on the kernel of this class (fewer launches).
+ * (8*nw band cells per lane) of the nibble kernels, 0 = as many as stay resident;
+ * merge_below: band classes narrower than this run on the kernel of this class
+ * (fewer launches).  The tile shape of the unit-cost kernels: tile_cands
+ * candidates per wave's tile (32 or 64), block_waves waves per block (1, 2 or 4),
+ * split_len: each band class splits into reads shorter than split_len and the
+ * rest (two launches, each with its own LDS staging size).
  */
 typedef struct srm_tuning {
     int blocks_per_cu[9];   /* by band class (8 diagonals per unit); 0 = as many as fit */
     int merge_below;        /* classes below this one run with it (0/1 = none) */
     int class_streams;      /* streams the band classes run on side by side (up to 8);
                                0 = default (4), 1 = one after another */
+    int tile_cands;         /* candidates per tile: 0 = 64, or 32 */
+    int block_waves;        /* waves per block of the unit-cost kernels: 0 = 4, or 1, 2 */
+    int split_len;          /* 0 = no split; else the read length that splits each class */
 } srm_tuning;
 int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *tuning);   /* NULL = defaults */
 /* after an eager srm_job_run: grid, tiles and event time (ms) of band class nw */

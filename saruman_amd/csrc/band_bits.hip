@@ -725,20 +725,27 @@ size_t bits_tile_words(int nw, int maxL)
     default: return (size_t)BitsShape<8>::rows(maxL) * 64;
     }
 }
+// waves per block of the unit-cost kernels: 1, 2 or 4 (anything else is 4)
+static int bits_block_waves(const BandArgs &a)
+{
+    return a.block_waves == 1 || a.block_waves == 2 ? a.block_waves : 4;
+}
+
 template <int NW>
 static hipError_t launch_bits_band_nw(const BandArgs &a, int blocks, size_t lds, hipStream_t st)
 {
-    hipLaunchKernelGGL(bits_band_kernel<NW>, dim3(blocks), dim3(256), lds, st, a);
+    hipLaunchKernelGGL(bits_band_kernel<NW>, dim3(blocks), dim3(64 * bits_block_waves(a)), lds, st, a);
     return hipGetLastError();
 }
 
 // reads of up to 250 codes stage their codes in LDS (19 KiB per block of 4 waves,
-// 8 blocks per CU); longer ones are read from read_words
+// 8 blocks per CU); longer ones are read from read_words.  blocks: of
+// a.block_waves waves, one wave per tile
 hipError_t launch_bits_band(const BandArgs &args, int nw, int blocks, int maxL, hipStream_t st)
 {
     BandArgs a = args;
     a.code_words = maxL <= 250 ? ((maxL + 15) >> 4) + 3 : 0;
-    const size_t lds = (size_t)a.code_words * 256 * 4;
+    const size_t lds = (size_t)a.code_words * 64 * bits_block_waves(a) * 4;
     switch (nw) {
     case 1: return launch_bits_band_nw<1>(a, blocks, lds, st);
     case 2: return launch_bits_band_nw<2>(a, blocks, lds, st);
@@ -779,7 +786,8 @@ int bits_band_occupancy(int nw)
 template <int NW>
 static hipError_t launch_bits_trace_nw(const BandArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(bits_trace_kernel<NW>, dim3((a.ntiles + 3) / 4), dim3(256), 0, st, a);
+    const int bw = bits_block_waves(a);
+    hipLaunchKernelGGL(bits_trace_kernel<NW>, dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), 0, st, a);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ struct BandArgs {
     uint32_t *ops;
     int code_words;                 // LDS words per lane staging the read's codes
                                     // (bits_band_kernel, trace_kernel; 0: read_words)
+    int block_waves = 4;            // waves per block of the unit-cost kernels (1, 2 or 4)
 };
 
 struct RecArgs {

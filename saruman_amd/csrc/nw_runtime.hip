@@ -176,13 +176,15 @@ struct srm_job {
     size_t mat_elems_band = 0, mat_elems_plain = 0;   // matrix elements a run of each path needs
     // timing
     hipEvent_t ev[4] = {};               // start, after band, after plain, after records
-    hipEvent_t cs[9] = {}, ce[9] = {};   // around each band-class launch
+    hipEvent_t cs[17] = {}, ce[17] = {};   // around each band-class launch (a class
+                                           // may be split in two by read length)
     hipEvent_t fork_ev = nullptr;        // concurrent classes: fork and joins
     hipEvent_t join_ev[8] = {};
     std::vector<hipStream_t> side;       // streams of concurrent band classes
     int class_streams = 0;               // at the last upload
     float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f, rec_ms = 0.f;
     float class_ms[9] = {};          // by NW (1..8), last eager run
+    int block_waves = 4;             // waves per block of the unit-cost kernels (tuning)
     // graph
     bool graph_enabled = false;
     hipGraph_t graph = nullptr;
@@ -318,6 +320,7 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.tile_scratch = (const uint64_t *)j->d_tile_scratch.p + bc.tile_begin;
             a.res = (srm_result *)j->d_res.p;
             a.ops = (uint32_t *)j->d_ops.p;
+            a.block_waves = j->block_waves;
             if (timed) HIP_TRY(hipEventRecord(j->cs[c], cst));
             HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, bc.maxL, cst));
             HIP_TRY(srm::launch_trace(a, bc.nw, j->unit, bc.maxL, cst));
@@ -495,6 +498,11 @@ int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *t)
         if (t->blocks_per_cu[i] < 0 || t->blocks_per_cu[i] > 64) return fail(SRM_EINVAL, "blocks_per_cu");
     if (t->merge_below < 0 || t->merge_below > 8) return fail(SRM_EINVAL, "merge_below");
     if (t->class_streams < 0 || t->class_streams > 8) return fail(SRM_EINVAL, "class_streams");
+    if (t->tile_cands != 0 && t->tile_cands != 32 && t->tile_cands != 64)
+        return fail(SRM_EINVAL, "tile_cands");
+    if (t->block_waves != 0 && t->block_waves != 1 && t->block_waves != 2 && t->block_waves != 4)
+        return fail(SRM_EINVAL, "block_waves");
+    if (t->split_len < 0 || t->split_len > 65535) return fail(SRM_EINVAL, "split_len");
     ctx->tuning = *t;
     return SRM_OK;
 }
@@ -506,9 +514,9 @@ int srm_job_band_info(srm_job *j, int nw, int *blocks, int *ntiles, float *ms)
     *ntiles = 0;
     *ms = j->class_ms[nw];
     for (const auto &bc : j->classes)
-        if (bc.nw == nw) {
-            *blocks = bc.blocks;
-            *ntiles = (int)bc.ntiles;
+        if (bc.nw == nw) {   // the two halves of a class split by read length add up
+            *blocks += bc.blocks;
+            *ntiles += (int)bc.ntiles;
         }
     return SRM_OK;
 }
@@ -597,7 +605,7 @@ void srm_job_destroy(srm_job *j)
         (void)hipStreamDestroy(q);
     }
     for (auto *arr : {&j->ev[0], &j->cs[0], &j->ce[0], &j->join_ev[0]}) {
-        const int cnt = arr == &j->ev[0] ? 4 : arr == &j->join_ev[0] ? 8 : 9;
+        const int cnt = arr == &j->ev[0] ? 4 : arr == &j->join_ev[0] ? 8 : 17;
         for (int i = 0; i < cnt; ++i)
             if (arr[i]) (void)hipEventDestroy(arr[i]);
     }
@@ -643,8 +651,11 @@ size_t srm_job_bytes_bound(const srm_ctx *ctx, int match, int mismatch, const sr
     size_t tiles = 0, scratch = 0;
     for (int nw = 1; nw <= 8; ++nw) {
         if (!n_cls[nw]) continue;
-        // every (class, L mod 8) run ends with at most one partial tile
-        const size_t tc = (n_cls[nw] + 63) / 64 + 8;
+        // every (class, L mod 8) run ends with at most one partial tile (and a
+        // class split by read length has twice the runs)
+        const size_t tcands = ctx && ctx->tuning.tile_cands == 32 ? 32 : 64;
+        const size_t runs = ctx && ctx->tuning.split_len > 0 ? 16 : 8;
+        const size_t tc = (n_cls[nw] + tcands - 1) / tcands + runs;
         tiles += tc;
         scratch += tc * (unit ? srm::bits_tile_words(nw, (int)maxL_cls[nw])
                               : tile_scratch_words(nw, (int)maxL_cls[nw]));
@@ -745,9 +756,16 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     // never mix L mod 8: a tile's lanes then cross the 8-column blocks of their
     // band scratch at the same traceback steps (a walk moves one column per
     // diagonal step), so the wave reloads band words together
+    // cls = 2 nw + (L >= split_len): a class split by read length (tuning) runs as
+    // two launches, the short reads with a smaller LDS staging size
     std::vector<int> cls(n, 0);
     const int merge_below = std::max(1, std::min(8, ctx->tuning.merge_below));
-    for (uint32_t t : band_idx) cls[t] = std::max((band_cells(cand[t]) + 7) / 8, merge_below);
+    const int split_len = ctx->tuning.split_len;
+    const uint32_t tile_cands = ctx->tuning.tile_cands == 32 ? 32u : 64u;
+    j->block_waves = ctx->tuning.block_waves ? ctx->tuning.block_waves : 4;
+    for (uint32_t t : band_idx)
+        cls[t] = 2 * std::max((band_cells(cand[t]) + 7) / 8, merge_below) +
+                 (split_len > 0 && (int)cand[t].L >= split_len ? 1 : 0);
     {
         // the order of a stable sort by (class, L mod 8, L descending), from two
         // stable counting passes: by L descending, then by (class, L mod 8)
@@ -758,7 +776,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         for (uint32_t t : band_idx) ++at[(size_t)(maxL - cand[t].L) + 1];
         for (size_t x = 1; x < at.size(); ++x) at[x] += at[x - 1];
         for (uint32_t t : band_idx) tmp[at[maxL - cand[t].L]++] = t;
-        std::vector<size_t> bk(9 * 8 + 1, 0);
+        std::vector<size_t> bk(18 * 8 + 1, 0);
         auto key = [&](uint32_t t) { return (size_t)cls[t] * 8 + (cand[t].L & 7); };
         for (uint32_t t : tmp) ++bk[key(t) + 1];
         for (size_t x = 1; x < bk.size(); ++x) bk[x] += bk[x - 1];
@@ -782,17 +800,18 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         size_t i = 0;
         while (i < band_idx.size()) {
             BandClass bc;
-            bc.nw = cls[band_idx[i]];
+            const int ci = cls[band_idx[i]];
+            bc.nw = ci / 2;
             bc.tile_begin = (uint32_t)(tiles.size() / 4);
             bc.maxL = 0;
             size_t e = i;
-            while (e < band_idx.size() && cls[band_idx[e]] == bc.nw) ++e;
+            while (e < band_idx.size() && cls[band_idx[e]] == ci) ++e;
             for (size_t g = i; g < e;) {
                 // one run of equal L mod 8
                 size_t ge = g;
                 while (ge < e && (cand[band_idx[ge]].L & 7) == (cand[band_idx[g]].L & 7)) ++ge;
-                for (size_t s = g; s < ge; s += 64) {
-                    const uint32_t cnt = (uint32_t)std::min<size_t>(64, ge - s);
+                for (size_t s = g; s < ge; s += tile_cands) {
+                    const uint32_t cnt = (uint32_t)std::min<size_t>(tile_cands, ge - s);
                     uint32_t kmax = 0, lmax = 0;
                     for (size_t u = s; u < s + cnt; ++u) {
                         kmax = std::max<uint32_t>(kmax, cand[band_idx[u]].k);
@@ -821,8 +840,8 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
                             sizeof(uint32_t) * 4 * bc.ntiles);
             }
             if (match == 0 && mismatch == 1) {
-                // bits_band_kernel: one wave per tile
-                bc.blocks = (int)((bc.ntiles + 3) / 4);
+                // bits_band_kernel: one wave per tile, blocks of block_waves waves
+                bc.blocks = (int)((bc.ntiles + j->block_waves - 1) / j->block_waves);
             } else {
                 // persistent grid: as many 4-wave blocks as stay resident on every
                 // CU (occupancy of this kernel instance), never more waves than tiles
@@ -966,7 +985,11 @@ int srm_job_run(srm_job *j, int kernel)
     for (auto &x : j->class_ms) x = 0.f;
     if (kernel != SRM_KERNEL_PLAIN)   // the class events are recorded only with band launches
         for (size_t c = 0; c < j->classes.size(); ++c)
-            HIP_TRY(hipEventElapsedTime(&j->class_ms[j->classes[c].nw], j->cs[c], j->ce[c]));
+        {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, j->cs[c], j->ce[c]));
+            j->class_ms[j->classes[c].nw] += ms;
+        }
     return SRM_OK;
 }
 

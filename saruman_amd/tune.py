@@ -1,14 +1,19 @@
 """Autotuner for the banded kernels' launch shape on this GPU.
 
-For costs other than match 0 / mismatch 1, and every band class (8*nw band
-cells per lane, nw = 1..8), it measures the persistent nibble kernel with
-1..max resident blocks per CU, each class timed with HIP events of its own
-launch.  (The unit-cost kernels launch one wave per tile: for them that sweep is
-skipped.)  Then it tries running the narrow classes on a wider kernel (fewer
-launches, fewer tails) and running the classes concurrently on 2 or 4 streams,
-timing whole steps.  The result is kept only if it beats the default launch by
-2%.  The winner is cached in saruman_amd/tuning/<arch>.json and applied with
-srm_ctx_set_tuning().
+Unit costs (match 0 / mismatch 1, every SARUMAN run): the tile shape and the
+launch order of the bit-parallel kernels, by coordinate descent over
+  class_streams   band classes side by side on 1, 2, 4 or 8 streams
+  block_waves     waves per block (1, 2, 4; one wave per tile)
+  tile_cands      candidates per tile (64, or 32 with half the lanes idle)
+  split_len       each band class split by read length at a quartile of the
+                  batch's lengths (two launches, the short reads with less LDS)
+  merge_below     narrow classes run on a wider class's kernel (fewer launches)
+each point timed as the median of whole band phases (HIP events).
+Other costs: the persistent nibble kernel with 1..max resident blocks per CU per
+band class, then merging and streams.  The result is kept only if it beats the
+default launch by 2% on a fresh measurement of both; the table -- with every
+point of the search -- is written to saruman_amd/tuning/<arch>.json and applied
+with srm_ctx_set_tuning().
 
     python -m saruman_amd.tune [--reads N]     (on the GPU; default: the 2M-read mixed workload)
 """
@@ -31,7 +36,15 @@ ARCH = "gfx950"
 
 class Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int * 9), ("merge_below", C.c_int),
-                ("class_streams", C.c_int)]
+                ("class_streams", C.c_int), ("tile_cands", C.c_int), ("block_waves", C.c_int),
+                ("split_len", C.c_int)]
+
+
+KNOBS = ("merge_below", "class_streams", "tile_cands", "block_waves", "split_len")
+
+
+def defaults() -> dict:
+    return {"blocks_per_cu": [0] * 9, **{k: 0 for k in KNOBS}}
 
 
 def table_path(arch: str = ARCH) -> Path:
@@ -54,8 +67,8 @@ def apply(ctx, table) -> None:
     t = Tuning()
     for i, b in enumerate(table["blocks_per_cu"][:9]):
         t.blocks_per_cu[i] = int(b)
-    t.merge_below = int(table.get("merge_below", 0))
-    t.class_streams = int(table.get("class_streams", 0))
+    for k in KNOBS:
+        setattr(t, k, int(table.get(k, 0)))
     _lib.check(lib.srm_ctx_set_tuning(ctx.h, C.byref(t)), "srm_ctx_set_tuning")
 
 
@@ -88,56 +101,95 @@ def _total(job, reps):
     return float(np.median([job.run("banded") for _ in range(reps)]))
 
 
-def tune(ctx, upload, reps: int = 3, log=print, unit: bool = True):
+def _search_unit(ctx, job, upload, reps, log, lengths):
+    """Coordinate descent over the unit-cost launch knobs; returns (best table,
+    every point measured)."""
+    q = [int(x) for x in np.quantile(lengths, (0.25, 0.5, 0.75))] if len(lengths) else []
+    space = [("class_streams", [4, 1, 2, 8]), ("block_waves", [4, 2, 1]),
+             ("tile_cands", [64, 32]), ("split_len", [0] + sorted(set(q))),
+             ("merge_below", [0, 2, 3])]
+    best = defaults()
+    points = []
+
+    def measure(t):
+        apply(ctx, t)
+        upload(job)
+        ms = _total(job, reps)
+        points.append({**{k: t[k] for k in KNOBS}, "band_ms": ms})
+        log(f"[tune] {' '.join(f'{k}={t[k]}' for k in KNOBS)}: {ms:.3f} ms")
+        return ms
+
+    best_ms = measure(best)
+    for _ in range(2):                       # two sweeps over the knobs
+        changed = False
+        for knob, values in space:
+            for v in values:
+                if v == best[knob] or (knob == "class_streams" and v == 4 and best[knob] == 0):
+                    continue
+                t = {**best, knob: v}
+                ms = measure(t)
+                if ms < best_ms:
+                    best, best_ms, changed = t, ms, True
+        if not changed:
+            break
+    return best, points
+
+
+def tune(ctx, upload, reps: int = 5, log=print, unit: bool = True, lengths=()):
     """upload(job) uploads the representative candidate batch into a Job; unit:
-    the batch runs match 0 / mismatch 1 (no blocks-per-CU sweep)."""
+    the batch runs match 0 / mismatch 1; lengths: its read lengths (split points)."""
     from .batch import Job
     lib = _lib.load()
     job = Job(ctx)
     try:
-        apply(ctx, {"blocks_per_cu": [0] * 9, "merge_below": 0, "class_streams": 1})
-        upload(job)
-        base_total, base = _class_times(job, reps)
-        log(f"[tune] default launch: band kernels {base_total:.3f} ms "
-            + " ".join(f"nw{nw}={ms:.3f}" for nw, ms in base.items()))
-        occ = {nw: lib.srm_band_occupancy(nw, 0) for nw in range(1, 9)}
-        best = {nw: (ms, 0) for nw, ms in base.items()}
-        for b in range(1, 0 if unit else max(occ.values()) + 1):
-            t = {"blocks_per_cu": [0] + [min(b, occ[nw]) for nw in range(1, 9)], "merge_below": 0,
-                 "class_streams": 1}
-            apply(ctx, t)
+        points = []
+        if unit:
+            cand, points = _search_unit(ctx, job, upload, reps, log, np.asarray(lengths))
+        else:
+            apply(ctx, {**defaults(), "class_streams": 1})
             upload(job)
-            tot, per = _class_times(job, reps)
-            log(f"[tune] {b} block(s)/CU: {tot:.3f} ms " +
-                " ".join(f"nw{nw}={ms:.3f}" for nw, ms in per.items()))
-            for nw, ms in per.items():
-                if b <= occ[nw] and ms < best.get(nw, (1e30, 0))[0]:
-                    best[nw] = (ms, b)
-        blocks = [0] + [best.get(nw, (0, 0))[1] for nw in range(1, 9)]
-        results = {}
-        for mb in (0, 2, 3):
-            for cs in (1, 2, 4):
-                t = {"blocks_per_cu": blocks, "merge_below": mb, "class_streams": cs}
+            base_total, base = _class_times(job, reps)
+            log(f"[tune] default launch: band kernels {base_total:.3f} ms "
+                + " ".join(f"nw{nw}={ms:.3f}" for nw, ms in base.items()))
+            occ = {nw: lib.srm_band_occupancy(nw, 0) for nw in range(1, 9)}
+            best = {nw: (ms, 0) for nw, ms in base.items()}
+            for b in range(1, max(occ.values()) + 1):
+                t = {**defaults(), "blocks_per_cu": [0] + [min(b, occ[nw]) for nw in range(1, 9)],
+                     "class_streams": 1}
                 apply(ctx, t)
                 upload(job)
-                tot = _total(job, reps)
-                results[(mb, cs)] = tot
-                log(f"[tune] merge_below={mb} class_streams={cs}: {tot:.3f} ms")
-        mb, cs = min(results, key=results.get)
+                tot, per = _class_times(job, reps)
+                log(f"[tune] {b} block(s)/CU: {tot:.3f} ms " +
+                    " ".join(f"nw{nw}={ms:.3f}" for nw, ms in per.items()))
+                for nw, ms in per.items():
+                    if b <= occ[nw] and ms < best.get(nw, (1e30, 0))[0]:
+                        best[nw] = (ms, b)
+            blocks = [0] + [best.get(nw, (0, 0))[1] for nw in range(1, 9)]
+            results = {}
+            for mb in (0, 2, 3):
+                for cs in (1, 2, 4):
+                    t = {**defaults(), "blocks_per_cu": blocks, "merge_below": mb,
+                         "class_streams": cs}
+                    apply(ctx, t)
+                    upload(job)
+                    results[(mb, cs)] = _total(job, reps)
+                    log(f"[tune] merge_below={mb} class_streams={cs}: {results[(mb, cs)]:.3f} ms")
+            mb, cs = min(results, key=results.get)
+            cand = {**defaults(), "blocks_per_cu": blocks, "merge_below": mb, "class_streams": cs}
         # keep the default launch unless the tuned one is at least 2% faster on a
         # fresh measurement of both
         apply(ctx, None)
         upload(job)
-        d_tot = _total(job, reps)
-        cand = {"blocks_per_cu": blocks, "merge_below": mb, "class_streams": cs}
+        d_tot = _total(job, 2 * reps)
         apply(ctx, cand)
         upload(job)
-        t_tot = _total(job, reps)
+        t_tot = _total(job, 2 * reps)
         log(f"[tune] default {d_tot:.3f} ms vs tuned {t_tot:.3f} ms")
         if t_tot > 0.98 * d_tot:
-            cand = {"blocks_per_cu": [0] * 9, "merge_below": 0, "class_streams": 0}
+            cand = defaults()
         table = {"arch": ARCH, **cand, "band_ms_default": d_tot, "band_ms_tuned": t_tot,
-                 "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S")}
+                 "kept": "tuned" if cand != defaults() else "defaults",
+                 "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"), "search": points}
         apply(ctx, table)
         return table
     finally:
@@ -162,7 +214,7 @@ def main():
     def upload(job):
         job.upload(0, 1, db.cand, db.read_words, None, ops_total=db.ops_total)
 
-    table = tune(ctx, upload)
+    table = tune(ctx, upload, lengths=db.L)
     table["workload"] = f"{args.workload}, {args.reads} reads, {db.n} candidates"
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     Path(args.out).write_text(json.dumps(table, indent=1) + "\n")

@@ -184,8 +184,9 @@ def test_graph_replay(orc):
 
 def test_tuned_launch_shapes_keep_results(orc):
     """Every launch shape the autotuner may pick gives the same results: grids,
-    merged classes, and classes run concurrently on several streams, eagerly and
-    replayed from a hipGraph."""
+    merged classes, classes run concurrently on several streams, tiles of 32
+    candidates, blocks of 1 or 2 waves and classes split by read length, eagerly
+    and replayed from a hipGraph."""
     banner("[gpu] tuning shapes")
     from saruman_amd import tune
     from saruman_amd.batch import Context, Job, _layout, pack_codes
@@ -207,7 +208,13 @@ def test_tuned_launch_shapes_keep_results(orc):
                              ({"blocks_per_cu": [0] * 9, "merge_below": 2, "class_streams": 2},
                               True),
                              ({"blocks_per_cu": [0] * 9, "merge_below": 0, "class_streams": 8},
-                              True)):
+                              True),
+                             ({"blocks_per_cu": [0] * 9, "class_streams": 4, "tile_cands": 32,
+                               "block_waves": 2}, False),
+                             ({"blocks_per_cu": [0] * 9, "class_streams": 2, "block_waves": 1,
+                               "split_len": 120}, True),
+                             ({"blocks_per_cu": [0] * 9, "merge_below": 2, "class_streams": 1,
+                               "tile_cands": 32, "split_len": 100}, False)):
             tune.apply(ctx, table)
             job = Job(ctx)
             job.upload(0, 1, lay["cand"], pack_codes(lay["rcat"]), pack_codes(lay["wcat"]),
@@ -230,5 +237,36 @@ def test_tuned_launch_shapes_keep_results(orc):
         for res, ops in outs[1:]:
             assert (res == outs[0][0]).all()
             assert (ops[used] == outs[0][1][used]).all()
+    finally:
+        ctx.close()
+
+
+def test_tile_shapes_other_costs(orc):
+    """Tiles of 32 candidates and classes split by read length on the nibble
+    kernels (costs other than match 0 / mismatch 1): the default launch's results."""
+    banner("[gpu] tuning shapes, other costs")
+    from saruman_amd import tune
+    from saruman_amd.batch import Context, Job, _layout, pack_codes
+    rng = np.random.default_rng(16)
+    batch = random_batch(rng, 2000, L=(30, 160), k=4, flags=(0, 1, 3, 5, 7), match=1,
+                         mismatch=3, per_cand_k=lambda L: 2 + L // 30)
+    lay = _layout(batch)
+    ctx = Context(0)
+    try:
+        outs = []
+        for table in (None, {"blocks_per_cu": [0] * 9, "tile_cands": 32, "split_len": 80,
+                             "class_streams": 2}):
+            tune.apply(ctx, table)
+            job = Job(ctx)
+            job.upload(batch.match, batch.mismatch, lay["cand"], pack_codes(lay["rcat"]),
+                       pack_codes(lay["wcat"]), ops_total=lay["ops_total"])
+            job.run("banded")
+            outs.append(job.download())
+            job.close()
+        want = orc.align_batch(batch)
+        k1 = batch.kvec() + 1
+        assert (np.minimum(outs[0][0]["score"].astype(np.int64), k1) ==
+                np.minimum(np.asarray(want.scores, np.int64), k1)).all()
+        assert (outs[1][0] == outs[0][0]).all()
     finally:
         ctx.close()
