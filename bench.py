@@ -62,7 +62,8 @@ def kernel_sha() -> str:
     return h.hexdigest()[:12]
 
 
-TUNING_KNOBS = ("merge_below", "class_streams", "tile_cands", "block_waves", "split_len")
+TUNING_KNOBS = ("merge_below", "class_streams", "tile_cands", "block_waves", "split_len",
+                "walk_layout")
 
 
 def tuning_key(tuning):
@@ -287,6 +288,8 @@ def main():
     ap.add_argument("--no-dfilter", action="store_true", help="skip the GPU q-gram filter leg")
     ap.add_argument("--host-filter", action="store_true",
                     help="run the host filter too when the device filter makes the candidates")
+    ap.add_argument("--walk-layout", type=int, default=None, choices=(0, 1),
+                    help="override the tuning's walk layout (0 stored D bits, 1 checkpoints)")
     ap.add_argument("--class-streams", type=int, default=None,
                     help="run the band classes concurrently on N streams (overrides the tuning)")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -344,11 +347,12 @@ def main():
         f"{len(perfect_all)} reads with perfect hits")
 
     tuning = None if args.no_tuning else tune.apply_cached(ctx)
-    if args.class_streams is not None:
-        tuning = dict(tuning or {"blocks_per_cu": [0] * 9, "merge_below": 0})
-        tuning["class_streams"] = args.class_streams
-        tune.apply(ctx, tuning)
-    log(f"[bench] launch tuning: {tuning if tuning else 'defaults'}")
+    for knob, val in (("class_streams", args.class_streams), ("walk_layout", args.walk_layout)):
+        if val is not None:
+            tuning = dict(tuning or tune.defaults())
+            tuning[knob] = val
+            tune.apply(ctx, tuning)
+    log(f"[bench] launch tuning: {tuning_key(tuning)}")
 
     # ---- everything resident: genome, and per batch a job with its candidates,
     # read codes and record inputs; one graph per job (band phase + records)

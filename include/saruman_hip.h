@@ -240,7 +240,9 @@ int srm_ctx_borrow_genome(srm_ctx *dst, const srm_ctx *src);
  * (fewer launches).  The tile shape of the unit-cost kernels: tile_cands
  * candidates per wave's tile (32 or 64), block_waves waves per block (1, 2 or 4),
  * split_len: each band class splits into reads shorter than split_len and the
- * rest (two launches, each with its own LDS staging size).
+ * rest (two launches, each with its own LDS staging size).  walk_layout: what
+ * the unit-cost forward pass leaves for the walk (1 cuts its HBM traffic; the
+ * walk then does the segment's bit-parallel steps again).
  */
 typedef struct srm_tuning {
     int blocks_per_cu[9];   /* by band class (8 diagonals per unit); 0 = as many as fit */
@@ -250,6 +252,9 @@ typedef struct srm_tuning {
     int tile_cands;         /* candidates per tile: 0 = 64, or 32 */
     int block_waves;        /* waves per block of the unit-cost kernels: 0 = 4, or 1, 2 */
     int split_len;          /* 0 = no split; else the read length that splits each class */
+    int walk_layout;        /* unit costs: 0 = the forward pass stores the D bits the walk
+                               reads; 1 = it stores the state of every 32nd column and the
+                               walk recomputes each 32-column segment from it */
 } srm_tuning;
 int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *tuning);   /* NULL = defaults */
 /* after an eager srm_job_run: grid, tiles and event time (ms) of band class nw */

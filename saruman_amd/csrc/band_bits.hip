@@ -67,6 +67,19 @@ struct BitsShape {
     __host__ __device__ static constexpr int read_row(int Lmax) { return tail_row(Lmax) + 2 * NW + 2; }
     __host__ __device__ static constexpr int win_row(int Lmax) { return read_row(Lmax) + ((Lmax + 31) >> 4) + 1; }
     __host__ __device__ static constexpr int rows(int Lmax) { return win_row(Lmax) + 2 * (NWD + ((Lmax + 30) >> 5) + 2); }
+    // checkpoint layout (the walk recomputes a 32-column segment from its first
+    // column's state): per segment g the state of column 32g -- band cell 0's
+    // value, the +1 and -1 delta vectors, WS words -- then the tail, read and
+    // window code rows as above
+    static constexpr int WS = 1 + 2 * NWD;
+    __host__ __device__ static constexpr int segs(int Lmax) { return Lmax >= 2 ? ((Lmax - 2) >> 5) + 1 : 0; }
+    __host__ __device__ static constexpr int ck_tail_row(int Lmax) { return segs(Lmax) * WS; }
+    __host__ __device__ static constexpr int ck_read_row(int Lmax) { return ck_tail_row(Lmax) + 2 * NW + 2; }
+    __host__ __device__ static constexpr int ck_win_row(int Lmax) { return ck_read_row(Lmax) + ((Lmax + 31) >> 4) + 1; }
+    __host__ __device__ static constexpr int ck_rows(int Lmax) { return ck_win_row(Lmax) + 2 * (NWD + ((Lmax + 30) >> 5) + 2); }
+    template <bool CK> __host__ __device__ static constexpr int tail_at(int Lmax) { return CK ? ck_tail_row(Lmax) : tail_row(Lmax); }
+    template <bool CK> __host__ __device__ static constexpr int read_at(int Lmax) { return CK ? ck_read_row(Lmax) : read_row(Lmax); }
+    template <bool CK> __host__ __device__ static constexpr int win_at(int Lmax) { return CK ? ck_win_row(Lmax) : win_row(Lmax); }
 };
 
 // word and bit offset of diagonal i's D byte among a block's D words
@@ -135,6 +148,79 @@ __device__ __forceinline__ void transpose8(uint32_t (&x)[8])
     }
 }
 
+// the band cells of column o + 1 whose characters differ: window code planes
+// ch0/ch1 (chunks of 32 codes from the column's band cell 0, shifted by o & 31)
+// against the read code o from rbuf (its 16-code word)
+template <int NWD>
+__device__ __forceinline__ void bits_ne(const uint32_t (&ch0)[NWD + 1], const uint32_t (&ch1)[NWD + 1],
+                                        uint32_t rbuf, int o, uint32_t (&ne)[NWD])
+{
+    const uint32_t sh = (uint32_t)(o & 31);
+    const uint32_t rsh = (uint32_t)(o & 15) * 2u;
+    // the read code's bits as all-zero / all-one masks
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_sbfe((int)rbuf, rsh, 1);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_sbfe((int)rbuf, rsh + 1u, 1);
+#pragma unroll
+    for (int m = 0; m < NWD; ++m) {
+        const uint32_t g0 = __builtin_amdgcn_alignbit(ch0[m + 1], ch0[m], sh);
+        const uint32_t g1 = __builtin_amdgcn_alignbit(ch1[m + 1], ch1[m], sh);
+        ne[m] = (g0 ^ r0) | (g1 ^ r1);
+    }
+}
+
+// one column of the recurrence: the previous column's deltas VP / VN and the
+// cells whose characters differ give the column's D0 words and (when live) its
+// deltas
+template <int NWD>
+__device__ __forceinline__ void bits_column(uint32_t (&VP)[NWD], uint32_t (&VN)[NWD],
+                                            const uint32_t (&ne)[NWD], uint32_t (&d0)[NWD], bool live)
+{
+    // previous column's deltas moved to this column's rows (cell i of column c - 1
+    // is cell i - 1 here); the cell below the band enters as +1
+    uint32_t vp[NWD], vn[NWD];
+    if (NWD == 1) {
+        vp[0] = __builtin_amdgcn_alignbit(1u, VP[0], 1);
+        vn[0] = VN[0] >> 1;
+    } else {
+        vp[0] = __builtin_amdgcn_alignbit(VP[NWD - 1], VP[0], 1);
+        vp[NWD - 1] = __builtin_amdgcn_alignbit(1u, VP[NWD - 1], 1);
+        vn[0] = __builtin_amdgcn_alignbit(VN[NWD - 1], VN[0], 1);
+        vn[NWD - 1] = VN[NWD - 1] >> 1;
+    }
+    // D0 = (((Eq & vp) + vp) ^ vp) | Eq | vn, the add carrying across words
+    if (NWD == 1) {
+        const uint32_t eqp = vp[0] & ~ne[0];
+        d0[0] = ((eqp + vp[0]) ^ vp[0]) | ~ne[0] | vn[0];
+    } else {
+        const uint64_t P = ((uint64_t)vp[NWD - 1] << 32) | vp[0];
+        const uint64_t E = ~(((uint64_t)ne[NWD - 1] << 32) | ne[0]);
+        const uint64_t N = ((uint64_t)vn[NWD - 1] << 32) | vn[0];
+        const uint64_t D = (((E & P) + P) ^ P) | E | N;
+        d0[0] = (uint32_t)D;
+        d0[NWD - 1] = (uint32_t)(D >> 32);
+    }
+    // horizontal deltas, then this column's vertical deltas; the cell above the
+    // band enters with a horizontal +1
+    uint32_t hp[NWD], hn[NWD];
+#pragma unroll
+    for (int m = 0; m < NWD; ++m) {
+        hp[m] = vn[m] | ~(d0[m] | vp[m]);
+        hn[m] = d0[m] & vp[m];
+    }
+    uint32_t xh[NWD], hns[NWD];
+    xh[0] = (hp[0] << 1) | 1u;
+    hns[0] = hn[0] << 1;
+    if (NWD == 2) {
+        xh[NWD - 1] = __builtin_amdgcn_alignbit(hp[NWD - 1], hp[0], 31);
+        hns[NWD - 1] = __builtin_amdgcn_alignbit(hn[NWD - 1], hn[0], 31);
+    }
+#pragma unroll
+    for (int m = 0; m < NWD; ++m) {
+        VN[m] = live ? xh[m] & d0[m] : VN[m];
+        VP[m] = live ? hns[m] | ~(d0[m] | xh[m]) : VP[m];
+    }
+}
+
 template <int NW>
 struct BitsWaves {
     static constexpr int value = NW <= 4 ? 8 : 6;
@@ -144,7 +230,7 @@ struct BitsWaves {
 // candidate per lane (the tile format of band_kernel).  Not persistent: the
 // column loop is short, and tiles taken from one global counter serialise on its
 // atomics; the workgroup dispatcher hands out the tiles (longest first) instead.
-template <int NW>
+template <int NW, bool CK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BitsWaves<NW>::value)))
 void bits_band_kernel(BandArgs a)
 {
@@ -184,12 +270,16 @@ void bits_band_kernel(BandArgs a)
         }
         int vt = k;   // value of band cell 0 (row -k: M = k)
         auto store_boundary = [&](int q) {
-            uint32_t *__restrict__ b = scr + (size_t)q * WB * 64 + lane;
-            __builtin_nontemporal_store((uint32_t)vt, &b[DW * 64]);
+            // checkpoint layout: the first column of every 32-column segment only
+            if (CK && (q & 3)) return;
+            // (the stored layout keeps it after the block's D words)
+            constexpr int o0 = CK ? 0 : DW;
+            uint32_t *__restrict__ b = scr + (size_t)(CK ? (q >> 2) * S::WS : q * WB) * 64 + lane;
+            __builtin_nontemporal_store((uint32_t)vt, &b[o0 * 64]);
 #pragma unroll
             for (int m = 0; m < NWD; ++m) {
-                __builtin_nontemporal_store(VP[m], &b[(DW + 1 + m) * 64]);
-                __builtin_nontemporal_store(VN[m], &b[(DW + 1 + NWD + m) * 64]);
+                __builtin_nontemporal_store(VP[m], &b[(o0 + 1 + m) * 64]);
+                __builtin_nontemporal_store(VN[m], &b[(o0 + 1 + NWD + m) * 64]);
             }
         };
 
@@ -199,8 +289,8 @@ void bits_band_kernel(BandArgs a)
         const int64_t gbase = (int64_t)cd.win_off + dlo;
         const int64_t rbase = (int64_t)cd.read_off;
         // the window codes also go to the tile's window rows (2 per chunk) for the walk
-        uint32_t *__restrict__ wrow = scr + (size_t)S::win_row(Lmax) * 64 + lane;
-        uint32_t *__restrict__ rrow = scr + (size_t)S::read_row(Lmax) * 64 + lane;
+        uint32_t *__restrict__ wrow = scr + (size_t)S::template win_at<CK>(Lmax) * 64 + lane;
+        uint32_t *__restrict__ rrow = scr + (size_t)S::template read_at<CK>(Lmax) * 64 + lane;
         uint32_t ch0[NWD + 1], ch1[NWD + 1];
 #pragma unroll
         for (int m = 0; m <= NWD; ++m) {
@@ -237,7 +327,9 @@ void bits_band_kernel(BandArgs a)
                 return __builtin_amdgcn_alignbit(lcodes[(wi + 1) * 64], lcodes[wi * 64],
                                                  (uint32_t)(p & 15) * 2u);
             }
-            return fetch16(a.read_words, rbase + o);
+            // a lane past its read (o >= L in a tile of longer reads) never uses
+            // these codes: it reads at its read's end, inside the padding
+            return fetch16(a.read_words, rbase + imin(o, L));
         };
         uint32_t rbuf = 0;
         // the next chunk of window codes, loaded one block ahead (before the block's
@@ -289,68 +381,11 @@ void bits_band_kernel(BandArgs a)
             for (int j = 0; j < 8; ++j) {
                 const int c = cb + j;
                 const bool live = decltype(every_live)::value || c < L;
-                {
-                    const int o = c - 1;   // code offset of the column
-                    const uint32_t sh = (uint32_t)(o & 31);
-                    const uint32_t rsh = (uint32_t)(o & 15) * 2u;
-                    // the read code's bits as all-zero / all-one masks
-                    const uint32_t r0 = (uint32_t)__builtin_amdgcn_sbfe((int)rbuf, rsh, 1);
-                    const uint32_t r1 = (uint32_t)__builtin_amdgcn_sbfe((int)rbuf, rsh + 1u, 1);
-                    uint32_t ne[NWD];   // band cells whose characters differ
+                uint32_t ne[NWD], d0[NWD];
+                bits_ne<NWD>(ch0, ch1, rbuf, c - 1, ne);
+                bits_column<NWD>(VP, VN, ne, d0, live);
 #pragma unroll
-                    for (int m = 0; m < NWD; ++m) {
-                        const uint32_t g0 = __builtin_amdgcn_alignbit(ch0[m + 1], ch0[m], sh);
-                        const uint32_t g1 = __builtin_amdgcn_alignbit(ch1[m + 1], ch1[m], sh);
-                        ne[m] = (g0 ^ r0) | (g1 ^ r1);
-                    }
-                    // previous column's deltas moved to this column's rows (cell i
-                    // of column c - 1 is cell i - 1 here); the cell below the band
-                    // enters as +1
-                    uint32_t vp[NWD], vn[NWD];
-                    if (NWD == 1) {
-                        vp[0] = __builtin_amdgcn_alignbit(1u, VP[0], 1);
-                        vn[0] = VN[0] >> 1;
-                    } else {
-                        vp[0] = __builtin_amdgcn_alignbit(VP[NWD - 1], VP[0], 1);
-                        vp[NWD - 1] = __builtin_amdgcn_alignbit(1u, VP[NWD - 1], 1);
-                        vn[0] = __builtin_amdgcn_alignbit(VN[NWD - 1], VN[0], 1);
-                        vn[NWD - 1] = VN[NWD - 1] >> 1;
-                    }
-                    // D0 = (((Eq & vp) + vp) ^ vp) | Eq | vn, the add carrying across words
-                    uint32_t d0[NWD];
-                    if (NWD == 1) {
-                        const uint32_t eqp = vp[0] & ~ne[0];
-                        d0[0] = ((eqp + vp[0]) ^ vp[0]) | ~ne[0] | vn[0];
-                    } else {
-                        const uint64_t P = ((uint64_t)vp[NWD - 1] << 32) | vp[0];
-                        const uint64_t E = ~(((uint64_t)ne[NWD - 1] << 32) | ne[0]);
-                        const uint64_t N = ((uint64_t)vn[NWD - 1] << 32) | vn[0];
-                        const uint64_t D = (((E & P) + P) ^ P) | E | N;
-                        d0[0] = (uint32_t)D;
-                        d0[NWD - 1] = (uint32_t)(D >> 32);
-                    }
-                    // horizontal deltas, then this column's vertical deltas; the
-                    // cell above the band enters with a horizontal +1
-                    uint32_t hp[NWD], hn[NWD];
-#pragma unroll
-                    for (int m = 0; m < NWD; ++m) {
-                        hp[m] = vn[m] | ~(d0[m] | vp[m]);
-                        hn[m] = d0[m] & vp[m];
-                    }
-                    uint32_t xh[NWD], hns[NWD];
-                    xh[0] = (hp[0] << 1) | 1u;
-                    hns[0] = hn[0] << 1;
-                    if (NWD == 2) {
-                        xh[NWD - 1] = __builtin_amdgcn_alignbit(hp[NWD - 1], hp[0], 31);
-                        hns[NWD - 1] = __builtin_amdgcn_alignbit(hn[NWD - 1], hn[0], 31);
-                    }
-#pragma unroll
-                    for (int m = 0; m < NWD; ++m) {
-                        VN[m] = live ? xh[m] & d0[m] : VN[m];
-                        VP[m] = live ? hns[m] | ~(d0[m] | xh[m]) : VP[m];
-                        d[j][m] = d0[m];
-                    }
-                }
+                for (int m = 0; m < NWD; ++m) d[j][m] = d0[m];
             }
             };
             if (__all(!active || cb + 8 <= L))
@@ -370,9 +405,14 @@ void bits_band_kernel(BandArgs a)
                     }
                 }
             }
-            // ---- block q: D bits by diagonal
-            uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
+            // ---- block q: D bits by diagonal (the checkpoint layout stores none: the
+            // walk recomputes them)
             uint32_t diag0 = 0;
+            if (CK) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) diag0 |= (d[j][0] & 1u) << j;
+            } else {
+            uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
 #pragma unroll
             for (int m = 0; m < NWD; ++m) {
                 uint32_t x[8];
@@ -394,6 +434,7 @@ void bits_band_kernel(BandArgs a)
                     __builtin_nontemporal_store(y, &blk[(base + u) * 64]);
                 }
             }
+            }
             // band cell 0 grows by 1 per column where its D0 bit is 0
             const int cnt = imax(imin(L - cb, 8), 0);
             vt += cnt - __builtin_popcount(diag0 & ((1u << cnt) - 1u));
@@ -403,7 +444,7 @@ void bits_band_kernel(BandArgs a)
         // diagonal (nibble j of word m is cell 8m + j, values clipped at K), then
         // the equal-character bits of column L.  Column L - 1 from its deltas;
         // column L with the free top moves of its trailing rows.
-        uint32_t *__restrict__ tail = scr + (size_t)S::tail_row(Lmax) * 64 + lane;
+        uint32_t *__restrict__ tail = scr + (size_t)S::template tail_at<CK>(Lmax) * 64 + lane;
         if (active) {
             const uint32_t rc = read16(L - 1) & 3u;
             const uint32_t rrep = rc * 0x55555555u;
@@ -456,8 +497,15 @@ void bits_band_kernel(BandArgs a)
 // Traceback of every hit of a unit-cost band class (cuda-functions.cu:504-534),
 // one wave per tile of bits_band_kernel.  The first run (column L's free rows)
 // and column 0 are as in trace_kernel; elsewhere the walk goes block by block.
-template <int NW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NW <= 4 ? 8 : 7)))
+// occupancy target of the walk: the checkpoint layout holds a segment's 32 D0
+// columns in registers (a lower target leaves room for them without spilling)
+template <int NW, bool CK>
+struct TraceWaves {
+    static constexpr int value = CK ? (NW <= 4 ? 4 : 3) : (NW <= 4 ? 8 : 7);
+};
+
+template <int NW, bool CK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TraceWaves<NW, CK>::value)))
 void bits_trace_kernel(BandArgs a)
 {
     using S = BitsShape<NW>;
@@ -473,7 +521,7 @@ void bits_trace_kernel(BandArgs a)
     // the packed tail (the first run's columns L - 1 and L), loaded with the
     // descriptor rather than after the score test
     const uint32_t *__restrict__ tail =
-        a.scratch + a.tile_scratch[tile] + (size_t)S::tail_row((int)td.z) * 64 + lane;
+        a.scratch + a.tile_scratch[tile] + (size_t)S::template tail_at<CK>((int)td.z) * 64 + lane;
     typedef uint32_t TailWords __attribute__((ext_vector_type(NW)));
     TailWords tl, tp;
 #pragma unroll
@@ -537,8 +585,58 @@ void bits_trace_kernel(BandArgs a)
     }
     // the read's codes and the band's window codes, as the forward pass left them
     // in the tile's rows: a segment's words are in the same few rows for the lanes
-    const uint32_t *__restrict__ rrow = scr + (size_t)S::read_row((int)td.z) * 64 + lane;
-    const uint32_t *__restrict__ wrow = scr + (size_t)S::win_row((int)td.z) * 64 + lane;
+    const uint32_t *__restrict__ rrow = scr + (size_t)S::template read_at<CK>((int)td.z) * 64 + lane;
+    const uint32_t *__restrict__ wrow = scr + (size_t)S::template win_at<CK>((int)td.z) * 64 + lane;
+    // checkpoint layout: the D0 words of the 32 columns of segment rg, recomputed
+    // from the segment's first column, and the state of its four blocks' first
+    // columns (band cell 0's value, the delta vectors)
+    uint32_t dcol[32][NWD];
+    int bv[4] = {0, 0, 0, 0};
+    uint32_t bp[4][NWD], bn[4][NWD];
+    int rg = -1;
+    auto recompute = [&](int g) {
+        const uint32_t *__restrict__ sb = scr + (size_t)g * S::WS * 64 + lane;
+        int v0 = (int)sb[0];
+        uint32_t VPr[NWD], VNr[NWD];
+#pragma unroll
+        for (int m = 0; m < NWD; ++m) {
+            VPr[m] = sb[(1 + m) * 64];
+            VNr[m] = sb[(1 + NWD + m) * 64];
+        }
+        // window code planes of chunks g .. g + NWD and the read codes of the
+        // segment, from the rows the forward pass left
+        uint32_t c0[NWD + 1], c1[NWD + 1];
+#pragma unroll
+        for (int m = 0; m <= NWD; ++m)
+            planes_of(wrow[(2 * (g + m)) * 64], wrow[(2 * (g + m) + 1) * 64], c0[m], c1[m]);
+        const uint32_t rw0 = rrow[(2 * g) * 64], rw1 = rrow[(2 * g + 1) * 64];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            if ((j & 7) == 0) {
+                bv[j >> 3] = v0;
+#pragma unroll
+                for (int m = 0; m < NWD; ++m) {
+                    bp[j >> 3][m] = VPr[m];
+                    bn[j >> 3][m] = VNr[m];
+                }
+            }
+            uint32_t ne[NWD];
+            bits_ne<NWD>(c0, c1, j < 16 ? rw0 : rw1, j, ne);
+            bits_column<NWD>(VPr, VNr, ne, dcol[j], true);
+            v0 += 1 - (int)(dcol[j][0] & 1u);
+        }
+    };
+    // D0 bits of diagonal di over the segment's columns (bit j: column 32 rg + 1 + j);
+    // 0 outside the band
+    auto gather = [&](int di) -> uint32_t {
+        if (di < 0 || di >= BMAX) return 0u;
+        const uint32_t b = (uint32_t)(di & 31);
+        const bool hi = NWD == 2 && di >= 32;
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) x |= ((hi ? dcol[j][NWD - 1] : dcol[j][0]) >> b & 1u) << j;
+        return x;
+    };
     // the walk goes by segments of 32 columns (4 blocks) along its diagonal: the D0
     // bits of the diagonal in columns 32g+1 .. 32g+32 and the mismatches of those
     // cells, bit j for column 32g + 1 + j; a diagonal run crosses a segment in one
@@ -617,14 +715,6 @@ void bits_trace_kernel(BandArgs a)
         const int g = (col - 1) >> 5;
         int j0 = (col - 1) & 31;
         if (i != ci || g != cg) {
-            // D words of blocks 4g .. 4g+3 (clamped to the read's last block: bits of
-            // columns past the walk's cell are never looked at)
-            int wd = 0, sh = 0;
-            dloc<NW>(imin(imax(i, 0), BMAX - 1), wd, sh);
-            uint32_t dw[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                dw[u] = scr[((size_t)imin(4 * g + u, qlast) * WB + wd) * 64 + lane];
             // characters: read codes 32g .. 32g+31 (rows 2g, 2g+1) against window
             // codes from band cell 0's code + i + 32g (exact for rows and columns
             // >= 1, the only cells whose characters can decide)
@@ -644,10 +734,32 @@ void bits_trace_kernel(BandArgs a)
             const uint32_t x0 = r0 ^ __builtin_amdgcn_alignbit(wg1, wg0, gsh);
             const uint32_t x1 = r1 ^ __builtin_amdgcn_alignbit(wg2, wg1, gsh);
             mseg = even_bits(x0 | (x0 >> 1)) | (even_bits(x1 | (x1 >> 1)) << 16);
-            dseg = 0u;
+            // the D0 bits of the diagonal over the segment decide only where the run
+            // below the walk's column meets differing characters: at equal ones D0
+            // is 1 (a unit-cost cell is its diagonal predecessor or one more, and
+            // equal characters give the predecessor).  A run without differing
+            // characters is walked as all ones, nothing loaded or recomputed.
+            if (i < 0 || i >= BMAX) {
+                dseg = 0u;
+            } else if (!(mseg & ((2u << j0) - 1u))) {
+                dseg = ~0u;
+            } else if (CK) {
+                if (g != rg) {
+                    recompute(g);
+                    rg = g;
+                }
+                dseg = gather(i);
+            } else {
+                // D words of blocks 4g .. 4g+3 (clamped to the read's last block: bits
+                // of columns past the walk's cell are never looked at)
+                int wd = 0, sh = 0;
+                dloc<NW>(i, wd, sh);
+                dseg = 0u;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) dseg |= ((dw[u] >> sh) & 0xFFu) << (8 * u);
-            if (i < 0 || i >= BMAX) dseg = 0u;
+                for (int u = 0; u < 4; ++u)
+                    dseg |= ((scr[((size_t)imin(4 * g + u, qlast) * WB + wd) * 64 + lane] >> sh) &
+                             0xFFu) << (8 * u);
+            }
             ci = i;
             cg = g;
         }
@@ -678,15 +790,34 @@ void bits_trace_kernel(BandArgs a)
         // one cell that is not on a diagonal run (col >= 1)
         const int q = (col - 1) >> 3, jb = (col - 1) & 7;
         if (i != si || q != sq) {
-            const uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
-            dm = dbyte(blk, i - 1);
-            dp = dbyte(blk, i + 1);
-            if (q != sq) {   // the boundary belongs to the block, not the diagonal
-                bvt = (int)blk[DW * 64];
+            if (CK) {
+                // block q is block u of the walk's segment, recomputed if it is not
+                // the one held
+                const int gq = q >> 2;
+                if (gq != rg) {
+                    recompute(gq);
+                    rg = gq;
+                }
+                const int u = q - 4 * rg;
+                dm = (gather(i - 1) >> (8 * u)) & 0xFFu;
+                dp = (gather(i + 1) >> (8 * u)) & 0xFFu;
+                bvt = u == 0 ? bv[0] : u == 1 ? bv[1] : u == 2 ? bv[2] : bv[3];
 #pragma unroll
                 for (int m = 0; m < NWD; ++m) {
-                    bvp[m] = blk[(DW + 1 + m) * 64];
-                    bvn[m] = blk[(DW + 1 + NWD + m) * 64];
+                    bvp[m] = u == 0 ? bp[0][m] : u == 1 ? bp[1][m] : u == 2 ? bp[2][m] : bp[3][m];
+                    bvn[m] = u == 0 ? bn[0][m] : u == 1 ? bn[1][m] : u == 2 ? bn[2][m] : bn[3][m];
+                }
+            } else {
+                const uint32_t *__restrict__ blk = scr + (size_t)q * WB * 64 + lane;
+                dm = dbyte(blk, i - 1);
+                dp = dbyte(blk, i + 1);
+                if (q != sq) {   // the boundary belongs to the block, not the diagonal
+                    bvt = (int)blk[DW * 64];
+#pragma unroll
+                    for (int m = 0; m < NWD; ++m) {
+                        bvp[m] = blk[(DW + 1 + m) * 64];
+                        bvn[m] = blk[(DW + 1 + NWD + m) * 64];
+                    }
                 }
             }
             si = i;
@@ -712,19 +843,27 @@ void bits_trace_kernel(BandArgs a)
 // host-side launch wrappers
 // ---------------------------------------------------------------------------
 // words of one tile's scratch for the unit-cost kernels
-size_t bits_tile_words(int nw, int maxL)
+// words of one tile's scratch for the unit-cost kernels (ck: checkpoint layout)
+template <int NW>
+static size_t bits_tile_words_nw(int maxL, bool ck)
+{
+    return (size_t)(ck ? BitsShape<NW>::ck_rows(maxL) : BitsShape<NW>::rows(maxL)) * 64;
+}
+
+size_t bits_tile_words(int nw, int maxL, bool ck)
 {
     switch (nw) {
-    case 1: return (size_t)BitsShape<1>::rows(maxL) * 64;
-    case 2: return (size_t)BitsShape<2>::rows(maxL) * 64;
-    case 3: return (size_t)BitsShape<3>::rows(maxL) * 64;
-    case 4: return (size_t)BitsShape<4>::rows(maxL) * 64;
-    case 5: return (size_t)BitsShape<5>::rows(maxL) * 64;
-    case 6: return (size_t)BitsShape<6>::rows(maxL) * 64;
-    case 7: return (size_t)BitsShape<7>::rows(maxL) * 64;
-    default: return (size_t)BitsShape<8>::rows(maxL) * 64;
+    case 1: return bits_tile_words_nw<1>(maxL, ck);
+    case 2: return bits_tile_words_nw<2>(maxL, ck);
+    case 3: return bits_tile_words_nw<3>(maxL, ck);
+    case 4: return bits_tile_words_nw<4>(maxL, ck);
+    case 5: return bits_tile_words_nw<5>(maxL, ck);
+    case 6: return bits_tile_words_nw<6>(maxL, ck);
+    case 7: return bits_tile_words_nw<7>(maxL, ck);
+    default: return bits_tile_words_nw<8>(maxL, ck);
     }
 }
+
 // waves per block of the unit-cost kernels: 1, 2 or 4 (anything else is 4)
 static int bits_block_waves(const BandArgs &a)
 {
@@ -734,7 +873,10 @@ static int bits_block_waves(const BandArgs &a)
 template <int NW>
 static hipError_t launch_bits_band_nw(const BandArgs &a, int blocks, size_t lds, hipStream_t st)
 {
-    hipLaunchKernelGGL(bits_band_kernel<NW>, dim3(blocks), dim3(64 * bits_block_waves(a)), lds, st, a);
+    if (a.walk_ck)
+        hipLaunchKernelGGL((bits_band_kernel<NW, true>), dim3(blocks), dim3(64 * bits_block_waves(a)), lds, st, a);
+    else
+        hipLaunchKernelGGL((bits_band_kernel<NW, false>), dim3(blocks), dim3(64 * bits_block_waves(a)), lds, st, a);
     return hipGetLastError();
 }
 
@@ -764,7 +906,7 @@ static int bits_occupancy_nw()
 {
     int blocks = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocks, (const void *)bits_band_kernel<NW>, 256, 0);
+        &blocks, (const void *)(bits_band_kernel<NW, false>), 256, 0);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
@@ -787,7 +929,10 @@ template <int NW>
 static hipError_t launch_bits_trace_nw(const BandArgs &a, hipStream_t st)
 {
     const int bw = bits_block_waves(a);
-    hipLaunchKernelGGL(bits_trace_kernel<NW>, dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), 0, st, a);
+    if (a.walk_ck)
+        hipLaunchKernelGGL((bits_trace_kernel<NW, true>), dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), 0, st, a);
+    else
+        hipLaunchKernelGGL((bits_trace_kernel<NW, false>), dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), 0, st, a);
     return hipGetLastError();
 }
 

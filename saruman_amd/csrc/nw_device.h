@@ -49,6 +49,8 @@ struct BandArgs {
     int code_words;                 // LDS words per lane staging the read's codes
                                     // (bits_band_kernel, trace_kernel; 0: read_words)
     int block_waves = 4;            // waves per block of the unit-cost kernels (1, 2 or 4)
+    int walk_ck = 0;                // unit costs: checkpoint layout (the walk recomputes
+                                    // each 32-column segment) instead of stored D bits
 };
 
 struct RecArgs {
@@ -101,7 +103,7 @@ hipError_t launch_trace(const BandArgs &a, int nw, bool unit, int maxL, hipStrea
 hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, int maxL, hipStream_t st);
 hipError_t launch_bits_trace(const BandArgs &a, int nw, hipStream_t st);
 int bits_band_occupancy(int nw);
-size_t bits_tile_words(int nw, int maxL);   // one tile's scratch (words)
+size_t bits_tile_words(int nw, int maxL, bool ck);   // one tile's scratch (words)
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);
 int band_occupancy(int nw, bool unit);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);

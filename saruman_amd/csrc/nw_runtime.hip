@@ -185,6 +185,7 @@ struct srm_job {
     float last_ms = 0.f, band_ms = 0.f, plain_ms = 0.f, rec_ms = 0.f;
     float class_ms[9] = {};          // by NW (1..8), last eager run
     int block_waves = 4;             // waves per block of the unit-cost kernels (tuning)
+    int walk_ck = 0;                 // unit costs: checkpoint scratch layout (tuning)
     // graph
     bool graph_enabled = false;
     hipGraph_t graph = nullptr;
@@ -321,6 +322,7 @@ int run_launches(srm_job *j, int kernel, bool timed)
             a.res = (srm_result *)j->d_res.p;
             a.ops = (uint32_t *)j->d_ops.p;
             a.block_waves = j->block_waves;
+            a.walk_ck = j->walk_ck;
             if (timed) HIP_TRY(hipEventRecord(j->cs[c], cst));
             HIP_TRY(srm::launch_band(a, bc.nw, j->unit, bc.blocks, bc.maxL, cst));
             HIP_TRY(srm::launch_trace(a, bc.nw, j->unit, bc.maxL, cst));
@@ -503,6 +505,7 @@ int srm_ctx_set_tuning(srm_ctx *ctx, const srm_tuning *t)
     if (t->block_waves != 0 && t->block_waves != 1 && t->block_waves != 2 && t->block_waves != 4)
         return fail(SRM_EINVAL, "block_waves");
     if (t->split_len < 0 || t->split_len > 65535) return fail(SRM_EINVAL, "split_len");
+    if (t->walk_layout != 0 && t->walk_layout != 1) return fail(SRM_EINVAL, "walk_layout");
     ctx->tuning = *t;
     return SRM_OK;
 }
@@ -657,7 +660,8 @@ size_t srm_job_bytes_bound(const srm_ctx *ctx, int match, int mismatch, const sr
         const size_t runs = ctx && ctx->tuning.split_len > 0 ? 16 : 8;
         const size_t tc = (n_cls[nw] + tcands - 1) / tcands + runs;
         tiles += tc;
-        scratch += tc * (unit ? srm::bits_tile_words(nw, (int)maxL_cls[nw])
+        scratch += tc * (unit ? srm::bits_tile_words(nw, (int)maxL_cls[nw],
+                                                     ctx && ctx->tuning.walk_layout == 1)
                               : tile_scratch_words(nw, (int)maxL_cls[nw]));
     }
     size_t b = 0;
@@ -763,6 +767,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     const int split_len = ctx->tuning.split_len;
     const uint32_t tile_cands = ctx->tuning.tile_cands == 32 ? 32u : 64u;
     j->block_waves = ctx->tuning.block_waves ? ctx->tuning.block_waves : 4;
+    j->walk_ck = ctx->tuning.walk_layout;
     for (uint32_t t : band_idx)
         cls[t] = 2 * std::max((band_cells(cand[t]) + 7) / 8, merge_below) +
                  (split_len > 0 && (int)cand[t].L >= split_len ? 1 : 0);
@@ -856,7 +861,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
                 tile_off.push_back(scratch_words);
                 const int tmax = (int)tiles[4 * (bc.tile_begin + u) + 2];
                 scratch_words += (match == 0 && mismatch == 1)
-                                     ? srm::bits_tile_words(bc.nw, tmax)
+                                     ? srm::bits_tile_words(bc.nw, tmax, j->walk_ck != 0)
                                      : tile_scratch_words(bc.nw, tmax);
             }
             j->classes.push_back(bc);

@@ -139,7 +139,7 @@ extern "C" int srm_align_static(int device, void *stream, int match, int mismatc
                 h_tiles[4 * ntiles + 2] = lmax;
                 h_tiles[4 * ntiles + 3] = kmax;
                 h_tile_off[ntiles] = words;
-                words += unit ? srm::bits_tile_words(bc.nw, (int)lmax)
+                words += unit ? srm::bits_tile_words(bc.nw, (int)lmax, false)
                               : srm::tile_scratch_words(bc.nw, (int)lmax);
                 bc.maxL = std::max(bc.maxL, (int)lmax);
                 ++ntiles;

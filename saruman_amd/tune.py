@@ -8,6 +8,8 @@ launch order of the bit-parallel kernels, by coordinate descent over
   split_len       each band class split by read length at a quartile of the
                   batch's lengths (two launches, the short reads with less LDS)
   merge_below     narrow classes run on a wider class's kernel (fewer launches)
+  walk_layout     the forward pass stores the D bits for the walk (0), or every
+                  32nd column's state from which the walk recomputes (1)
 each point timed as the median of whole band phases (HIP events).
 Other costs: the persistent nibble kernel with 1..max resident blocks per CU per
 band class, then merging and streams.  The result is kept only if it beats the
@@ -37,10 +39,10 @@ ARCH = "gfx950"
 class Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int * 9), ("merge_below", C.c_int),
                 ("class_streams", C.c_int), ("tile_cands", C.c_int), ("block_waves", C.c_int),
-                ("split_len", C.c_int)]
+                ("split_len", C.c_int), ("walk_layout", C.c_int)]
 
 
-KNOBS = ("merge_below", "class_streams", "tile_cands", "block_waves", "split_len")
+KNOBS = ("merge_below", "class_streams", "tile_cands", "block_waves", "split_len", "walk_layout")
 
 
 def defaults() -> dict:
@@ -105,7 +107,7 @@ def _search_unit(ctx, job, upload, reps, log, lengths):
     """Coordinate descent over the unit-cost launch knobs; returns (best table,
     every point measured)."""
     q = [int(x) for x in np.quantile(lengths, (0.25, 0.5, 0.75))] if len(lengths) else []
-    space = [("class_streams", [4, 1, 2, 8]), ("block_waves", [4, 2, 1]),
+    space = [("walk_layout", [0, 1]), ("class_streams", [4, 1, 2, 8]), ("block_waves", [4, 2, 1]),
              ("tile_cands", [64, 32]), ("split_len", [0] + sorted(set(q))),
              ("merge_below", [0, 2, 3])]
     best = defaults()

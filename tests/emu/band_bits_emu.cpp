@@ -4,7 +4,8 @@
 // oracle (oracle/nw_oracle.c: oracle_fill, oracle_traceback).  TEST INFRASTRUCTURE
 // ONLY.
 //
-// usage: band_bits_emu N [seed [plant]] -> one summary line, exit status 1 on a mismatch.
+// usage: band_bits_emu N [seed [plant]] -> one summary line per scratch layout (stored D
+// bits, checkpoints), exit status 1 on a mismatch.
 // "plant" reads the oracle's gap rows the wrong way round (top for left), a mistake
 // planted in the checker, which the comparison must report.
 #include <algorithm>
@@ -69,7 +70,7 @@ int oracle_ops(const Case &c, const std::vector<char> &rch, const std::vector<ch
 }
 
 template <int NW>
-void run_class(std::vector<Case> &cs, Tally &tally)
+void run_class(std::vector<Case> &cs, Tally &tally, bool ck)
 {
     const int n = (int)cs.size();
     if (!n) return;
@@ -126,7 +127,7 @@ void run_class(std::vector<Case> &cs, Tally &tally)
         maxL = imx(maxL, mL);
         tiles.push_back(uint4{(uint32_t)b, (uint32_t)cnt, (uint32_t)mL, (uint32_t)mk});
         toff.push_back(words);
-        words += srm::bits_tile_words(NW, mL);
+        words += srm::bits_tile_words(NW, mL, ck);
     }
     std::vector<srm_cand> ctile(n);
     for (int p = 0; p < n; ++p) ctile[p] = cand[perm[p]];
@@ -149,6 +150,7 @@ void run_class(std::vector<Case> &cs, Tally &tally)
     a.tile_scratch = toff.data();
     a.res = res.data();
     a.ops = ops.data();
+    a.walk_ck = ck;
     const uint32_t blocks = (a.ntiles + 3) / 4;
     // forward pass: the LDS staging rule of launch_bits_band
     a.code_words = maxL <= 250 ? ((maxL + 15) >> 4) + 3 : 0;
@@ -157,7 +159,10 @@ void run_class(std::vector<Case> &cs, Tally &tally)
             threadIdx.x = th;
             blockIdx.x = blk;
             blockDim.x = 256;
-            srm::bits_band_kernel<NW>(a);
+            if (ck)
+                srm::bits_band_kernel<NW, true>(a);
+            else
+                srm::bits_band_kernel<NW, false>(a);
         }
     // walk (codes from the tile's rows)
     a.code_words = 0;
@@ -166,7 +171,10 @@ void run_class(std::vector<Case> &cs, Tally &tally)
             threadIdx.x = th;
             blockIdx.x = blk;
             blockDim.x = 256;
-            srm::bits_trace_kernel<NW>(a);
+            if (ck)
+                srm::bits_trace_kernel<NW, true>(a);
+            else
+                srm::bits_trace_kernel<NW, false>(a);
         }
     std::vector<int16_t> M;
     for (int t = 0; t < n; ++t) {
@@ -189,8 +197,9 @@ void run_class(std::vector<Case> &cs, Tally &tally)
         }
         if (!ok) {
             if (tally.bad < 5)
-                fprintf(stderr, "mismatch NW%d L=%d w=%d k=%d flag=%d stop=%d: score %d (oracle %d)\n",
-                        NW, c.L, c.w, c.k, c.flag, c.stop, res[t].score, sc);
+                fprintf(stderr, "mismatch NW%d%s L=%d w=%d k=%d flag=%d stop=%d: score %d (oracle %d)\n",
+                        NW, ck ? " (checkpoint layout)" : "", c.L, c.w, c.k, c.flag, c.stop,
+                        res[t].score, sc);
             ++tally.bad;
         }
     }
@@ -248,18 +257,22 @@ int main(int argc, char **argv)
         c.stop = rnd() % 5 == 0;
         by[nw][L > 250].push_back(c);
     }
-    Tally tally;
-    for (int g = 0; g < 2; ++g) {
-        run_class<1>(by[1][g], tally);
-        run_class<2>(by[2][g], tally);
-        run_class<3>(by[3][g], tally);
-        run_class<4>(by[4][g], tally);
-        run_class<5>(by[5][g], tally);
-        run_class<6>(by[6][g], tally);
-        run_class<7>(by[7][g], tally);
-        run_class<8>(by[8][g], tally);
-    }
-    printf("cases %d hits %ld bad %ld diag %ld top %ld left %ld\n", N, tally.hits, tally.bad,
-           tally.ops[0], tally.ops[1], tally.ops[2]);
-    return tally.bad != 0;
+    // both scratch layouts: stored D bits, and checkpoints the walk recomputes from
+    Tally tally[2];
+    for (int ck = 0; ck < 2; ++ck)
+        for (int g = 0; g < 2; ++g) {
+            run_class<1>(by[1][g], tally[ck], ck);
+            run_class<2>(by[2][g], tally[ck], ck);
+            run_class<3>(by[3][g], tally[ck], ck);
+            run_class<4>(by[4][g], tally[ck], ck);
+            run_class<5>(by[5][g], tally[ck], ck);
+            run_class<6>(by[6][g], tally[ck], ck);
+            run_class<7>(by[7][g], tally[ck], ck);
+            run_class<8>(by[8][g], tally[ck], ck);
+        }
+    for (int ck = 0; ck < 2; ++ck)
+        printf("%s: cases %d hits %ld bad %ld diag %ld top %ld left %ld\n",
+               ck ? "checkpoint" : "stored", N, tally[ck].hits, tally[ck].bad, tally[ck].ops[0],
+               tally[ck].ops[1], tally[ck].ops[2]);
+    return tally[0].bad != 0 || tally[1].bad != 0;
 }

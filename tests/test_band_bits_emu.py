@@ -1,5 +1,7 @@
 """The unit-cost band kernels (saruman_amd/csrc/band_bits.hip: bits_band_kernel and
-bits_trace_kernel), compiled unmodified for the CPU through a stand-in HIP header
+bits_trace_kernel, in both scratch layouts: stored D bits, and the checkpoints
+from which the walk recomputes each 32-column segment), compiled unmodified for
+the CPU through a stand-in HIP header
 (tests/emu/hip/hip_runtime.h) and run lane by lane over random candidates of every
 band class, against the oracle (oracle_fill and oracle_traceback, which restate
 cuda-functions.cu:413-497 and :108-315): score, transcript and end cell of every
@@ -46,22 +48,29 @@ def emu_bin(tmp_path_factory):
 
 
 def _run(exe, n, seed, plant=False):
+    """Runs the emulation; returns (exit status, {layout: tallies}) for the two
+    scratch layouts (stored D bits, checkpoints the walk recomputes from)."""
     args = [str(exe), str(n), str(seed)] + (["plant"] if plant else [])
-    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
-    m = re.search(r"cases (\d+) hits (\d+) bad (\d+) diag (\d+) top (\d+) left (\d+)", r.stdout)
-    assert m, r.stdout + r.stderr
-    return r.returncode, dict(zip(("cases", "hits", "bad", "diag", "top", "left"),
-                                  map(int, m.groups())))
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600)
+    out = {}
+    for m in re.finditer(r"(stored|checkpoint): cases (\d+) hits (\d+) bad (\d+) diag (\d+) "
+                         r"top (\d+) left (\d+)", r.stdout):
+        out[m.group(1)] = dict(zip(("cases", "hits", "bad", "diag", "top", "left"),
+                                   map(int, m.groups()[1:])))
+    assert set(out) == {"stored", "checkpoint"}, r.stdout + r.stderr
+    return r.returncode, out
 
 
 @pytest.mark.parametrize("seed", [1, 0x5A2F])
 def test_bits_kernels_match_oracle(emu_bin, seed):
-    rc, t = _run(emu_bin, 20000, seed)
-    assert rc == 0 and t["bad"] == 0, t
-    # the cases reach every kind of step
-    assert t["hits"] > 5000 and t["top"] > 1000 and t["left"] > 1000, t
+    rc, out = _run(emu_bin, 20000, seed)
+    assert rc == 0, out
+    for layout, t in out.items():
+        assert t["bad"] == 0, (layout, t)
+        # the cases reach every kind of step
+        assert t["hits"] > 5000 and t["top"] > 1000 and t["left"] > 1000, (layout, t)
 
 
 def test_planted_mistake_is_reported(emu_bin):
-    rc, t = _run(emu_bin, 4000, 3, plant=True)
-    assert rc == 1 and t["bad"] > 0, t
+    rc, out = _run(emu_bin, 4000, 3, plant=True)
+    assert rc == 1 and out["stored"]["bad"] > 0 and out["checkpoint"]["bad"] > 0, out

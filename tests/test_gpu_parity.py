@@ -185,8 +185,8 @@ def test_graph_replay(orc):
 def test_tuned_launch_shapes_keep_results(orc):
     """Every launch shape the autotuner may pick gives the same results: grids,
     merged classes, classes run concurrently on several streams, tiles of 32
-    candidates, blocks of 1 or 2 waves and classes split by read length, eagerly
-    and replayed from a hipGraph."""
+    candidates, blocks of 1 or 2 waves, classes split by read length and the
+    checkpoint scratch layout, eagerly and replayed from a hipGraph."""
     banner("[gpu] tuning shapes")
     from saruman_amd import tune
     from saruman_amd.batch import Context, Job, _layout, pack_codes
@@ -214,7 +214,12 @@ def test_tuned_launch_shapes_keep_results(orc):
                              ({"blocks_per_cu": [0] * 9, "class_streams": 2, "block_waves": 1,
                                "split_len": 120}, True),
                              ({"blocks_per_cu": [0] * 9, "merge_below": 2, "class_streams": 1,
-                               "tile_cands": 32, "split_len": 100}, False)):
+                               "tile_cands": 32, "split_len": 100}, False),
+                             ({"blocks_per_cu": [0] * 9, "walk_layout": 1}, False),
+                             ({"blocks_per_cu": [0] * 9, "walk_layout": 1, "class_streams": 2,
+                               "block_waves": 1, "merge_below": 2}, True),
+                             ({"blocks_per_cu": [0] * 9, "walk_layout": 1, "tile_cands": 32,
+                               "split_len": 120, "class_streams": 8}, False)):
             tune.apply(ctx, table)
             job = Job(ctx)
             job.upload(0, 1, lay["cand"], pack_codes(lay["rcat"]), pack_codes(lay["wcat"]),
