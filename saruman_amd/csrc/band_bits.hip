@@ -644,7 +644,7 @@ void bits_trace_kernel(BandArgs a)
     int ci = -1, cg = -1;
     uint32_t dseg = 0, mseg = 0;
     uint32_t r0 = 0, r1 = 0;   // read codes of segment cg (rows 2cg, 2cg + 1)
-    int wl0 = -1;              // window row of wg0 (wg0 .. wg2: three window rows from it)
+    int wl0 = INT32_MIN;       // window row of wg0 (wg0 .. wg2: three window rows from it); none yet
     uint32_t wg0 = 0, wg1 = 0, wg2 = 0;
     const int qlast = (L - 2) >> 3;   // last block holding columns of this read
     // the side cells of a non-diagonal step (block sq of diagonal si): D0 bits of
