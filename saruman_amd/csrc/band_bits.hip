@@ -482,13 +482,8 @@ void bits_band_kernel(BandArgs a)
             }
             tail[(2 * NW) * 64] = eqw[0];
             tail[(2 * NW + 1) * 64] = eqw[1];
-            // ---- score (cuda-functions.cu:495-497); the walk runs in bits_trace_kernel
-            srm_result r;
-            r.score = (int16_t)score;
-            r.n_ops = 0;
-            r.end_row = (uint16_t)w;
-            r.end_col = (uint16_t)L;
-            a.res[a.perm[pos]] = r;
+            // ---- score (cuda-functions.cu:495-497), in tile order: bits_trace_kernel
+            // writes every candidate's result (one scattered write per candidate)
             a.tile_score[pos] = score;
         }
     }
@@ -531,16 +526,31 @@ void bits_trace_kernel(BandArgs a)
     }
     const uint32_t eq0 = tail[(2 * NW) * 64], eq1 = tail[(2 * NW + 1) * 64];
     const int k = cd.k, K = k + 1;
-    if (score > k) return;
     const uint32_t t = a.perm[pos];
     srm_result r;
     r.score = (int16_t)score;
     const int L = cd.L, w = cd.w;
+    if (score > k) {   // no walk: the score and the walk's start cell
+        r.n_ops = 0;
+        r.end_row = (uint16_t)w;
+        r.end_col = (uint16_t)L;
+        a.res[t] = r;
+        return;
+    }
     const int s = flag_s(cd.flag, k), e = flag_e(cd.flag, k);
     const int dlo = -k;
     const uint32_t *__restrict__ scr = a.scratch + a.tile_scratch[tile];
     const bool stop_row0 = (cd.bits & SRM_CAND_STOP_ROW0) != 0;
     uint32_t *__restrict__ ops = a.ops + cd.ops_off;
+    // the transcript words go to LDS as they complete (lane-interleaved), and to
+    // the slot in one burst at the end, so that the L2 merges each slot's sectors
+    // (a.stage_words words per lane; 0: straight to the slot)
+    extern __shared__ uint32_t walk_lds[];
+    uint32_t *__restrict__ stage = walk_lds + (size_t)(threadIdx.x >> 6) * 64 * a.stage_words + lane;
+    auto put_word = [&](int x, uint32_t v) {
+        if (a.stage_words) stage[x * 64] = v;
+        else ops[x] = v;
+    };
     const int cap = w + k;
     int row = w, col = L, n = 0;
     uint32_t opw = 0;
@@ -550,7 +560,7 @@ void bits_trace_kernel(BandArgs a)
         opw |= (uint32_t)op << ((n & 15) * 2);
         ++n;
         if ((n & 15) == 0) {
-            ops[(n >> 4) - 1] = opw;
+            put_word((n >> 4) - 1, opw);
             opw = 0;
         }
         row -= (op != 2);
@@ -694,7 +704,7 @@ void bits_trace_kernel(BandArgs a)
                 row -= c;
                 T -= c;
                 if ((n & 15) == 0) {
-                    ops[(n >> 4) - 1] = opw;
+                    put_word((n >> 4) - 1, opw);
                     opw = 0;
                 }
             }
@@ -775,9 +785,9 @@ void bits_trace_kernel(BandArgs a)
             v -= __builtin_popcount(~dseg & run);
             const int n2 = n + rr;
             if ((n2 >> 4) != (n >> 4)) {   // the run completes the current ops word
-                ops[n >> 4] = opw;
+                put_word(n >> 4, opw);
                 opw = 0;
-                if ((n2 >> 4) > (n >> 4) + 1) ops[(n >> 4) + 1] = 0u;   // and a whole one
+                if ((n2 >> 4) > (n >> 4) + 1) put_word((n >> 4) + 1, 0u);   // and a whole one
             }
             n = n2;
             row -= rr;
@@ -832,7 +842,9 @@ void bits_trace_kernel(BandArgs a)
         v = op == 0 ? Dg : op == 1 ? Tp : Lf;
         emit(op);
     }
-    if (n & 15) ops[n >> 4] = opw;
+    if (n & 15) put_word(n >> 4, opw);
+    if (a.stage_words)
+        for (int x = 0; x < ((n + 15) >> 4); ++x) ops[x] = stage[x * 64];
     r.n_ops = (uint16_t)n;
     r.end_row = (uint16_t)row;
     r.end_col = (uint16_t)col;
@@ -929,16 +941,25 @@ template <int NW>
 static hipError_t launch_bits_trace_nw(const BandArgs &a, hipStream_t st)
 {
     const int bw = bits_block_waves(a);
+    const size_t lds = (size_t)a.stage_words * 64 * bw * 4;
     if (a.walk_ck)
-        hipLaunchKernelGGL((bits_trace_kernel<NW, true>), dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), 0, st, a);
+        hipLaunchKernelGGL((bits_trace_kernel<NW, true>), dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), lds, st, a);
     else
-        hipLaunchKernelGGL((bits_trace_kernel<NW, false>), dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), 0, st, a);
+        hipLaunchKernelGGL((bits_trace_kernel<NW, false>), dim3((a.ntiles + bw - 1) / bw), dim3(64 * bw), lds, st, a);
     return hipGetLastError();
 }
 
 // the walk reads the codes from the tile's rows: no LDS
-hipError_t launch_bits_trace(const BandArgs &a, int nw, hipStream_t st)
+// transcript words per lane the walk stages in LDS for reads of up to maxL codes:
+// a band candidate's transcript holds at most w + k <= L + 63 ops (band cells
+// k + max(2s + k, w - L) + 1 <= 64), one more word for a run's lookahead write;
+// none for longer reads (the LDS would cut the walk's occupancy)
+int bits_trace_stage_words(int maxL) { return maxL <= 320 ? ((maxL + 63) >> 4) + 2 : 0; }
+
+hipError_t launch_bits_trace(const BandArgs &args, int nw, int maxL, hipStream_t st)
 {
+    BandArgs a = args;
+    a.stage_words = bits_trace_stage_words(maxL);
     switch (nw) {
     case 1: return launch_bits_trace_nw<1>(a, st);
     case 2: return launch_bits_trace_nw<2>(a, st);

@@ -51,6 +51,8 @@ struct BandArgs {
     int block_waves = 4;            // waves per block of the unit-cost kernels (1, 2 or 4)
     int walk_ck = 0;                // unit costs: checkpoint layout (the walk recomputes
                                     // each 32-column segment) instead of stored D bits
+    int stage_words = 0;            // unit-cost walk: transcript words per lane staged in
+                                    // LDS (0: written straight to the slot)
 };
 
 struct RecArgs {
@@ -101,7 +103,8 @@ hipError_t launch_band(const BandArgs &a, int nw, bool unit, int blocks, int max
 hipError_t launch_trace(const BandArgs &a, int nw, bool unit, int maxL, hipStream_t st);
 // unit costs (band_bits.hip): the bit-parallel forward pass and its walk
 hipError_t launch_bits_band(const BandArgs &a, int nw, int blocks, int maxL, hipStream_t st);
-hipError_t launch_bits_trace(const BandArgs &a, int nw, hipStream_t st);
+hipError_t launch_bits_trace(const BandArgs &a, int nw, int maxL, hipStream_t st);
+int bits_trace_stage_words(int maxL);   // LDS transcript words per lane for a class's walk
 int bits_band_occupancy(int nw);
 size_t bits_tile_words(int nw, int maxL, bool ck);   // one tile's scratch (words)
 hipError_t launch_reset(uint32_t *counters, int n, hipStream_t st);

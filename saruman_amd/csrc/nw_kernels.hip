@@ -649,7 +649,7 @@ static int trace_code_words(int maxL) { return ((maxL + 15) >> 4) + 3; }
 hipError_t launch_trace(const BandArgs &args, int nw, bool unit, int maxL, hipStream_t st)
 {
     if (args.ntiles == 0) return hipSuccess;
-    if (unit) return launch_bits_trace(args, nw, st);
+    if (unit) return launch_bits_trace(args, nw, maxL, st);
     BandArgs a = args;
     a.code_words = maxL <= 600 ? trace_code_words(maxL) : 0;
     const size_t lds = (size_t)a.code_words * 256 * 4;

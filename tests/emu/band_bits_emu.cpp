@@ -19,6 +19,7 @@ thread_local EmuIdx threadIdx, blockIdx, blockDim;
 namespace srm {
 uint32_t band_lds[256 * 64];    // one 256-thread block's LDS (the kernels' staging)
 uint32_t trace_lds[256 * 64];
+uint32_t walk_lds[256 * 64];
 }
 
 extern "C" int oracle_fill(int ma, int mm, int k, const char *win, int w, const char *read, int L,
@@ -164,8 +165,9 @@ void run_class(std::vector<Case> &cs, Tally &tally, bool ck)
             else
                 srm::bits_band_kernel<NW, false>(a);
         }
-    // walk (codes from the tile's rows)
+    // walk (codes from the tile's rows; transcript words staged as the launch does)
     a.code_words = 0;
+    a.stage_words = srm::bits_trace_stage_words(maxL);
     for (uint32_t blk = 0; blk < blocks; ++blk)
         for (uint32_t th = 0; th < 256; ++th) {
             threadIdx.x = th;
