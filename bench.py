@@ -40,23 +40,32 @@ PEAK_INT32_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz, int32 VALU ops/s
 PEAK_HBM_GBS = 8000.0
 OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 
-# HBM bytes of one step of the band phase (bits_band_kernel and bits_trace_kernel of every
-# class), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.000 and x1.000
-# by a 256 MiB copy kernel), for the exact configuration profiled: workload,
-# candidates, launch tuning and the sha256 prefix of the kernel sources.
+# Counters of one eager step of the exact configuration profiled (workload,
+# candidates, launch tuning and the sha256 prefix of the kernel sources), from
+# rocprofv3 --pmc runs with the kernels one after another:
+#   band_bytes      HBM bytes of the band phase (bits_band_kernel and
+#                   bits_trace_kernel of every class): FETCH_SIZE x2.000 + WRITE_SIZE
+#                   x1.000, the factors a 256 MiB copy kernel reads
+#   band_valu_busy  the band phase's VALU busy fraction, weighted by kernel time
+#                   (SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE per kernel)
+#   records_bytes   HBM bytes of the record kernels (rec_measure, rec_write_lines,
+#                   rec_write)
 # Source: profiles/bits_band_and_trace_kernels_mixed_2M_reads.md
-PMC_TRAFFIC = [
-    {"workload": "mixed", "candidates": 1774238, "tuning": "defaults",
-     "kernel_sha": "144bad373c94", "bytes": 4343037562},
+PMC_PROFILE = [
+    {"workload": "mixed", "candidates": 1774238,
+     "tuning": {"blocks_per_cu": [0, 0, 0, 0, 0, 0, 0, 0, 0], "merge_below": 2, "class_streams": 0, "tile_cands": 64, "block_waves": 1, "split_len": 0, "walk_layout": 1},
+     "kernel_sha": "b2e0e15e9101", "band_bytes": 1517920082, "band_valu_busy": 0.552,
+     "records_bytes": 1327580715},
 ]
 
 
 def kernel_sha() -> str:
-    """sha256 prefix of the alignment kernels' sources (both band kernel files and
-    their shared helpers)."""
+    """sha256 prefix of the kernels' sources the profiled counters depend on (both
+    band kernel files, the record kernels and their shared helpers)."""
     import hashlib
     h = hashlib.sha256()
-    for name in ("band_bits.hip", "nw_kernels.hip", "nw_seq.h"):
+    for name in ("band_bits.hip", "nw_kernels.hip", "nw_seq.h", "nw_device.h", "records_gpu.hip",
+                 "rec_ops.h"):
         with open(os.path.join(ROOT, "saruman_amd", "csrc", name), "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:12]
@@ -74,13 +83,14 @@ def tuning_key(tuning):
     return {k: tuning.get(k, 0) for k in ("blocks_per_cu",) + TUNING_KNOBS}
 
 
-def profiled_traffic(workload, candidates, tuning):
+def profiled(workload, candidates, tuning):
+    """The PMC_PROFILE entry of this exact configuration, or None."""
     tun = tuning_key(tuning)
     sha = kernel_sha()
-    for e in PMC_TRAFFIC:
+    for e in PMC_PROFILE:
         if (e["workload"] == workload and e["candidates"] == candidates and e["tuning"] == tun
                 and e["kernel_sha"] == sha):
-            return e["bytes"]
+            return e
     return None
 
 
@@ -375,11 +385,13 @@ def main():
     mapped = []
     hits_total = 0
     aligned_reads = 0
+    hit_ops = 0
     for (lo, hi, db), job in zip(batches, jobs):
         job.run("banded")
         res, _ = job.download()
         hit = res["score"].astype(np.int64) <= db.k
         hits_total += int(hit.sum())
+        hit_ops += int(res["n_ops"][hit].astype(np.int64).sum())
         al = np.unique(db.read_id[hit])
         aligned_reads += len(al)
         mapped.append(len(np.union1d(al, db.extra["perfect_read"])))
@@ -492,6 +504,14 @@ def main():
     achieved = OPS_PER_BAND_CELL * bcells / (band_ms_step / 1e3) / 1e12
     hbm = float(hbm_bytes(L, W, K).sum()) / steps_per_pass
 
+    prof = profiled(args.workload, int(n_cand), tuning)
+    band_bytes = prof["band_bytes"] if prof else None
+    # record stage, algorithmic bytes per step: the text written, the transcripts
+    # read (2 bits per op) with the 2-bit codes of both rows (at most 4 bits per op),
+    # and 48 bytes of per-candidate inputs (descriptor, result, position, id,
+    # length, offset)
+    rec_alg = (text_bytes + 0.75 * hit_ops + 48.0 * n_cand) / B
+
     cpu = None
     if not args.no_cpu:
         lo0, _, db0 = batches[0]
@@ -530,13 +550,27 @@ def main():
         "band_gcups": bcells / (band_ms_step / 1e3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS,
                      "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS,
-                     "traffic": profiled_traffic(args.workload, int(n_cand), tuning),
+                     "traffic": band_bytes,
                      "kernel": "bits_band_kernel + bits_trace_kernel of every band class of one "
                                "step (HIP events around the band phase); the dominant stage",
+                     "work": "6 int32 ops per band cell of the cell-wise recurrence; the "
+                             "bit-parallel kernels evaluate 32 cells per lane-op, so frac can "
+                             "exceed 1: valu_busy is the measured utilisation",
                      "ops_per_step": OPS_PER_BAND_CELL * bcells,
-                     "kernel_ms": band_ms_step},
+                     "kernel_ms": band_ms_step,
+                     "valu_busy": prof["band_valu_busy"] if prof else None,
+                     "hbm_frac_measured": (band_bytes / (band_ms_step / 1e3) / 1e9 / PEAK_HBM_GBS)
+                     if band_bytes else None,
+                     "traffic_over_algorithmic": band_bytes / hbm if band_bytes else None},
         "roofline_hbm": {"achieved": hbm / (band_ms_step / 1e3) / 1e9, "peak": PEAK_HBM_GBS,
                          "unit": "GB/s", "algorithmic_bytes": hbm},
+        "roofline_records": {"bound": "hbm", "achieved": rec_alg / (rec_ms_step / 1e3) / 1e9,
+                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": rec_alg / (rec_ms_step / 1e3) / 1e9 / PEAK_HBM_GBS,
+                             "traffic": prof["records_bytes"] if prof else None,
+                             "algorithmic_bytes": rec_alg, "kernel_ms": rec_ms_step,
+                             "kernel": "rec_measure + line-offset scan + rec_write_lines + "
+                                       "rec_write (HIP events around the record stage)"},
         "records": {"text_bytes_per_pass": text_bytes, "device_equals_host": bool(identical),
                     "host_format_s": t_host_fmt, "host_threads": 16},
         "plain": plain,
