@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -382,6 +383,83 @@ int run_launches(srm_job *j, int kernel, bool timed)
     return SRM_OK;
 }
 
+// out[p] = cand[perm[p]], p < n: the banded descriptors in tile order, gathered
+// from the uploaded descriptors (perm holds indexes below their count)
+__global__ void gather_cand_kernel(const srm_cand *__restrict__ cand,
+                                   const uint32_t *__restrict__ perm, srm_cand *__restrict__ out,
+                                   size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+        out[p] = cand[perm[p]];
+}
+
+hipError_t launch_gather_cand(const srm_cand *cand, const uint32_t *perm, srm_cand *out, size_t n,
+                              hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(gather_cand_kernel, dim3((unsigned)blocks), dim3(256), 0, st, cand, perm,
+                       out, n);
+    return hipGetLastError();
+}
+
+// host threads for a planning pass over n items: one below 200k, else up to 8
+unsigned plan_threads(size_t n)
+{
+    if (n < 200000) return 1;
+    return std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+}
+
+// fn(u, lo, hi) on T contiguous ranges of [0, n), range u on thread u
+template <class Fn>
+void par_ranges(unsigned T, size_t n, Fn fn)
+{
+    if (T <= 1) {
+        fn(0u, (size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (unsigned u = 0; u < T; ++u) th.emplace_back(fn, u, n * u / T, n * (u + 1) / T);
+    for (auto &x : th) x.join();
+}
+
+// stable sort of kv by bits [lo, lo + bits) of each element: least-significant-
+// digit counting passes of at most 11 bits, each split over T threads (a thread
+// counts its range's digits, then scatters that range to the offsets its counts
+// give, so equal digits keep their order)
+void radix_sort_kv(std::vector<uint64_t> &kv, int lo, int bits, unsigned T)
+{
+    const size_t n = kv.size();
+    const int passes = (bits + 10) / 11;
+    if (n < 2 || passes == 0) return;
+    const int db = (bits + passes - 1) / passes;
+    const size_t B = (size_t)1 << db;
+    std::vector<uint64_t> tmp(n);
+    std::vector<size_t> cnt((size_t)T * B);
+    for (int pass = 0; pass < passes; ++pass) {
+        const int sh = lo + pass * db;
+        const uint64_t mask = B - 1;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        par_ranges(T, n, [&](unsigned u, size_t a, size_t b) {
+            size_t *c = cnt.data() + (size_t)u * B;
+            for (size_t i = a; i < b; ++i) ++c[(kv[i] >> sh) & mask];
+        });
+        size_t run = 0;
+        for (size_t d = 0; d < B; ++d)
+            for (unsigned u = 0; u < T; ++u) {
+                const size_t c = cnt[(size_t)u * B + d];
+                cnt[(size_t)u * B + d] = run;
+                run += c;
+            }
+        par_ranges(T, n, [&](unsigned u, size_t a, size_t b) {
+            size_t *c = cnt.data() + (size_t)u * B;
+            for (size_t i = a; i < b; ++i) tmp[c[(kv[i] >> sh) & mask]++] = kv[i];
+        });
+        kv.swap(tmp);
+    }
+}
 }  // namespace
 
 namespace srm {
@@ -708,49 +786,95 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     j->resident_genome = (win_words == nullptr);
     const bool band_ok = match >= 0 && mismatch >= 1;
 
-    // ---- validate and classify
-    std::vector<uint32_t> band_idx, rest_idx, bytes_idx;
-    size_t ops_end = 0;
+    // ---- validate and classify, over contiguous ranges of candidates on several
+    // host threads; the lists keep index order (ranges concatenated in order) and
+    // the first failing candidate (lowest index) is the one reported
     const uint64_t read_codes = (uint64_t)n_read_words * 16;
     const uint64_t win_codes =
         j->resident_genome ? (uint64_t)ctx->genome_words * 16 : (uint64_t)n_win_words * 16;
     const uint64_t pad = SRM_CODE_PAD;
-    for (int t = 0; t < n; ++t) {
-        const srm_cand &c = cand[t];
-        if (c.L < 1 || c.L > SRM_MAX_L || c.w > SRM_MAX_W)
-            return fail(SRM_EINVAL, "candidate %d: L=%d w=%d outside 1..%d / 0..%d", t, c.L, c.w,
-                        SRM_MAX_L, SRM_MAX_W);
-        size_t oe = (size_t)c.ops_off + srm_ops_words(c.w, c.k);
-        ops_end = std::max(ops_end, oe);
+    enum { K_BAND, K_REST, K_BYTES, E_RANGE, E_BYTES, E_READ, E_GENOME, E_WIN };
+    auto classify = [&](const srm_cand &c) -> int {
+        if (c.L < 1 || c.L > SRM_MAX_L || c.w > SRM_MAX_W) return E_RANGE;
         if (c.bits & SRM_CAND_BYTES) {
             if ((uint64_t)c.read_off + c.L > n_read_bytes || (uint64_t)c.win_off + c.w > n_win_bytes)
-                return fail(SRM_EINVAL, "candidate %d: byte range out of bounds", t);
-            bytes_idx.push_back((uint32_t)t);
-            continue;
+                return E_BYTES;
+            return K_BYTES;
         }
         // code arrays: pad codes readable before 0 and after the end (the caller's
         // arrays start with SRM_CODE_PAD pad codes for uploaded arrays, and offsets
         // count from the first code after them).  Written without subtraction: an
         // array shorter than the padding must fail, not wrap.
-        if ((uint64_t)c.read_off + c.L + 2 * pad > read_codes)
-            return fail(SRM_EINVAL, "candidate %d: read codes out of bounds", t);
+        if ((uint64_t)c.read_off + c.L + 2 * pad > read_codes) return E_READ;
         if (j->resident_genome) {
             // the genome carries SRM_CODE_PAD codes of padding at both ends
             if ((uint64_t)c.win_off < pad || (uint64_t)c.win_off + c.w + pad > win_codes)
-                return fail(SRM_EINVAL, "candidate %d: genome window out of bounds", t);
+                return E_GENOME;
         } else if ((uint64_t)c.win_off + c.w + 2 * pad > win_codes) {
-            return fail(SRM_EINVAL, "candidate %d: window codes out of bounds", t);
+            return E_WIN;
         }
         // banded: exact-band conditions, and w >= L - k (a shorter window can never
         // hold a hit; the window code stream reads up to L + 93 codes past win_off)
         if (band_ok && c.k <= SRM_BAND_KMAX && band_cells(c) <= SRM_BAND_MAX &&
             (int)c.w >= (int)c.L - (int)c.k)
-            band_idx.push_back((uint32_t)t);
-        else
-            rest_idx.push_back((uint32_t)t);
-    }
+            return K_BAND;
+        return K_REST;
+    };
+    const unsigned TV = plan_threads((size_t)n);
+    struct Part {
+        std::vector<uint32_t> band, rest, bytes;
+        size_t ops_end = 0, rows = 0;
+        uint32_t maxL = 0;   // over the banded candidates
+        int bad = -1, why = 0;
+    };
+    std::vector<Part> part(TV);
+    par_ranges(TV, (size_t)n, [&](unsigned u, size_t a, size_t b) {
+        Part &P = part[u];
+        for (size_t t = a; t < b; ++t) {
+            const srm_cand &c = cand[t];
+            const int r = classify(c);
+            if (r >= E_RANGE) {
+                P.bad = (int)t;
+                P.why = r;
+                return;
+            }
+            P.ops_end = std::max(P.ops_end, (size_t)c.ops_off + srm_ops_words(c.w, c.k));
+            P.rows += 2 * ((size_t)c.w + c.k);
+            if (r == K_BAND) P.maxL = std::max<uint32_t>(P.maxL, c.L);
+            (r == K_BAND ? P.band : r == K_REST ? P.rest : P.bytes).push_back((uint32_t)t);
+        }
+    });
+    std::vector<uint32_t> band_idx, rest_idx, bytes_idx;
+    size_t ops_end = 0;
     j->rec_rows_bound = 0;
-    for (int t = 0; t < n; ++t) j->rec_rows_bound += 2 * ((size_t)cand[t].w + cand[t].k);
+    for (const Part &P : part) {
+        if (P.bad >= 0) {
+            const srm_cand &c = cand[P.bad];
+            switch (P.why) {
+            case E_RANGE:
+                return fail(SRM_EINVAL, "candidate %d: L=%d w=%d outside 1..%d / 0..%d", P.bad,
+                            c.L, c.w, SRM_MAX_L, SRM_MAX_W);
+            case E_BYTES: return fail(SRM_EINVAL, "candidate %d: byte range out of bounds", P.bad);
+            case E_READ: return fail(SRM_EINVAL, "candidate %d: read codes out of bounds", P.bad);
+            case E_GENOME:
+                return fail(SRM_EINVAL, "candidate %d: genome window out of bounds", P.bad);
+            default: return fail(SRM_EINVAL, "candidate %d: window codes out of bounds", P.bad);
+            }
+        }
+        ops_end = std::max(ops_end, P.ops_end);
+        j->rec_rows_bound += P.rows;
+    }
+    if (TV == 1) {
+        band_idx.swap(part[0].band);
+        rest_idx.swap(part[0].rest);
+        bytes_idx.swap(part[0].bytes);
+    } else {
+        for (const Part &P : part) {
+            band_idx.insert(band_idx.end(), P.band.begin(), P.band.end());
+            rest_idx.insert(rest_idx.end(), P.rest.begin(), P.rest.end());
+            bytes_idx.insert(bytes_idx.end(), P.bytes.begin(), P.bytes.end());
+        }
+    }
     j->n_band = band_idx.size();
     j->n_rest = rest_idx.size();
     j->n_bytes = bytes_idx.size();
@@ -762,31 +886,41 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     // diagonal step), so the wave reloads band words together
     // cls = 2 nw + (L >= split_len): a class split by read length (tuning) runs as
     // two launches, the short reads with a smaller LDS staging size
-    std::vector<int> cls(n, 0);
     const int merge_below = std::max(1, std::min(8, ctx->tuning.merge_below));
     const int split_len = ctx->tuning.split_len;
     const uint32_t tile_cands = ctx->tuning.tile_cands == 32 ? 32u : 64u;
     j->block_waves = ctx->tuning.block_waves ? ctx->tuning.block_waves : 4;
     j->walk_ck = ctx->tuning.walk_layout;
-    for (uint32_t t : band_idx)
-        cls[t] = 2 * std::max((band_cells(cand[t]) + 7) / 8, merge_below) +
-                 (split_len > 0 && (int)cand[t].L >= split_len ? 1 : 0);
-    {
-        // the order of a stable sort by (class, L mod 8, L descending), from two
-        // stable counting passes: by L descending, then by (class, L mod 8)
-        uint32_t maxL = 0;
-        for (uint32_t t : band_idx) maxL = std::max<uint32_t>(maxL, cand[t].L);
-        std::vector<uint32_t> tmp(band_idx.size());
-        std::vector<size_t> at((size_t)maxL + 2, 0);
-        for (uint32_t t : band_idx) ++at[(size_t)(maxL - cand[t].L) + 1];
-        for (size_t x = 1; x < at.size(); ++x) at[x] += at[x - 1];
-        for (uint32_t t : band_idx) tmp[at[maxL - cand[t].L]++] = t;
-        std::vector<size_t> bk(18 * 8 + 1, 0);
-        auto key = [&](uint32_t t) { return (size_t)cls[t] * 8 + (cand[t].L & 7); };
-        for (uint32_t t : tmp) ++bk[key(t) + 1];
-        for (size_t x = 1; x < bk.size(); ++x) bk[x] += bk[x - 1];
-        for (uint32_t t : tmp) band_idx[bk[key(t)]++] = t;
-    }
+    const size_t nb = band_idx.size();
+    const unsigned TB = plan_threads(nb);
+    // sort word of every banded candidate, in index order: its index (bits 0-31),
+    // k (bits 32-35) and from bit 36 the key: class (5 bits), L mod 8 (3 bits),
+    // then maxL - L in lb bits (L descending), the fields of the order above from
+    // the top down
+    uint32_t maxL = 0;
+    for (const Part &P : part) maxL = std::max(maxL, P.maxL);
+    int lb = 1;
+    while ((1u << lb) <= maxL) ++lb;
+    std::vector<uint64_t> kv(nb);
+    par_ranges(TB, nb, [&](unsigned, size_t a, size_t b) {
+        for (size_t i = a; i < b; ++i) {
+            const srm_cand &c = cand[band_idx[i]];
+            const uint32_t cl = 2u * (uint32_t)std::max((band_cells(c) + 7) / 8, merge_below) +
+                                (split_len > 0 && (int)c.L >= split_len ? 1u : 0u);
+            const uint32_t key = (((cl << 3) | ((uint32_t)c.L & 7u)) << lb) | (maxL - c.L);
+            kv[i] = ((uint64_t)key << 36) | ((uint64_t)c.k << 32) | band_idx[i];
+        }
+    });
+    static_assert(SRM_BAND_KMAX < 16, "k of a banded candidate fits 4 bits");
+    radix_sort_kv(kv, 36, 8 + lb, TB);
+    par_ranges(TB, nb, [&](unsigned, size_t a, size_t b) {
+        for (size_t i = a; i < b; ++i) band_idx[i] = (uint32_t)kv[i];
+    });
+    // by position in tile order
+    auto cls_at = [&](size_t i) { return (int)(kv[i] >> (36 + lb + 3)); };
+    auto lmod_at = [&](size_t i) { return (int)((kv[i] >> (36 + lb)) & 7u); };
+    auto k_at = [&](size_t i) { return (uint32_t)(kv[i] >> 32) & 15u; };
+    auto L_at = [&](size_t i) { return maxL - ((uint32_t)(kv[i] >> 36) & ((1u << lb) - 1u)); };
     std::vector<uint32_t> tiles;   // uint4 per tile
     j->classes.clear();
     size_t scratch_words = 0;
@@ -805,22 +939,22 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         size_t i = 0;
         while (i < band_idx.size()) {
             BandClass bc;
-            const int ci = cls[band_idx[i]];
+            const int ci = cls_at(i);
             bc.nw = ci / 2;
             bc.tile_begin = (uint32_t)(tiles.size() / 4);
             bc.maxL = 0;
             size_t e = i;
-            while (e < band_idx.size() && cls[band_idx[e]] == ci) ++e;
+            while (e < band_idx.size() && cls_at(e) == ci) ++e;
             for (size_t g = i; g < e;) {
                 // one run of equal L mod 8
                 size_t ge = g;
-                while (ge < e && (cand[band_idx[ge]].L & 7) == (cand[band_idx[g]].L & 7)) ++ge;
+                while (ge < e && lmod_at(ge) == lmod_at(g)) ++ge;
                 for (size_t s = g; s < ge; s += tile_cands) {
                     const uint32_t cnt = (uint32_t)std::min<size_t>(tile_cands, ge - s);
                     uint32_t kmax = 0, lmax = 0;
                     for (size_t u = s; u < s + cnt; ++u) {
-                        kmax = std::max<uint32_t>(kmax, cand[band_idx[u]].k);
-                        lmax = std::max<uint32_t>(lmax, cand[band_idx[u]].L);
+                        kmax = std::max(kmax, k_at(u));
+                        lmax = std::max(lmax, L_at(u));
                     }
                     tiles.push_back((uint32_t)s);
                     tiles.push_back(cnt);
@@ -891,15 +1025,13 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     if ((rc = upload(ctx, j->d_read_bytes, read_bytes, read_bytes ? n_read_bytes : 0))) return rc;
     if ((rc = upload(ctx, j->d_win_bytes, win_bytes, win_bytes ? n_win_bytes : 0))) return rc;
     if ((rc = upload(ctx, j->d_perm, band_idx.data(), band_idx.size()))) return rc;
-    {
-        // the banded candidates in tile order: a wave reads its tile's descriptors
-        // as one contiguous run, without the indirection through perm
-        std::vector<srm_cand> ct(band_idx.size());
-        for (size_t p = 0; p < band_idx.size(); ++p) ct[p] = cand[band_idx[p]];
-        if ((rc = upload(ctx, j->d_cand_tile, ct.data(), ct.size()))) return rc;
-        if ((rc = ctx->ensure(j->d_tile_score, std::max<size_t>(band_idx.size() * 4, 64))))
-            return rc;
-    }
+    // the banded candidates in tile order (a wave reads its tile's descriptors as
+    // one contiguous run, without the indirection through perm), gathered on the
+    // device from the uploaded descriptors
+    if ((rc = ctx->ensure(j->d_cand_tile, band_idx.size() * sizeof(srm_cand) + 64))) return rc;
+    HIP_TRY(launch_gather_cand((const srm_cand *)j->d_cand.p, (const uint32_t *)j->d_perm.p,
+                               (srm_cand *)j->d_cand_tile.p, band_idx.size(), ctx->stream));
+    if ((rc = ctx->ensure(j->d_tile_score, std::max<size_t>(band_idx.size() * 4, 64)))) return rc;
     if ((rc = upload(ctx, j->d_tiles, tiles.data(), tiles.size()))) return rc;
     if ((rc = upload(ctx, j->d_tile_scratch, tile_off.data(), tile_off.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_rest, rest_idx.data(), rest_idx.size()))) return rc;
