@@ -382,23 +382,76 @@ __device__ __forceinline__ uint32_t fields4(const uint32_t *w, int f)
     return (uint32_t)(v >> (2 * (f & 15))) & 0xFFu;
 }
 
+// four 2-bit fields (x < 256) spread into the four bytes, field t in byte t
+__device__ __forceinline__ uint32_t spread4(uint32_t x)
+{
+    const uint32_t t = (x | (x << 12)) & 0x000F000Fu;
+    return (t | (t << 6)) & 0x03030303u;
+}
+
+// the four characters of one row at four positions: gaps has 1 in byte t when
+// position t is a gap in this row ('_'), codes8 the row's next four codes (code
+// k in field k); position t prints code t - (gaps before t)
+__device__ __forceinline__ uint32_t row_chars4(uint32_t codes8, uint32_t gaps)
+{
+    const uint32_t ch = __builtin_amdgcn_perm(0u, 0x54474341u, spread4(codes8));   // "ACGT"
+    // gaps before byte t, t <= 3: bytes 0..2 of gaps summed into the bytes above
+    const uint32_t before = __umul24(gaps, 0x010101u) << 8;
+    // selector byte t: code byte t - before (0..3) or, for a gap, byte 4 of the
+    // pair = '_'
+    const uint32_t sel = 0x03020100u + (gaps << 2) - before;
+    return __builtin_amdgcn_perm(0x5F5F5F5Fu, ch, sel);
+}
+
+// a row's four characters c that start at slot byte x into the slot's words:
+// the word of x gets c's first 4 - (x & 3) bytes, after the last (x & 3) bytes
+// of prev (the four characters before) when pfull, else alone as bytes
+__device__ __forceinline__ void put_head(uint32_t *s32, int x, uint32_t c, uint32_t prev, bool pfull)
+{
+    const int d = x >> 2, sh = x & 3;
+    if (sh == 0 || pfull) {
+        s32[d] = sh == 0 ? c : __builtin_amdgcn_alignbyte(c, prev, (uint32_t)(4 - sh));
+    } else {
+        char *s8 = (char *)s32 + x;
+        for (int b = 0; b < 4 - sh; ++b) s8[b] = (char)(c >> (8 * b));
+    }
+}
+
+// the last (x & 3) bytes of the four characters c that start at slot byte x
+__device__ __forceinline__ void put_tail(uint32_t *s32, int x, uint32_t c)
+{
+    const int sh = x & 3;
+    char *s8 = (char *)s32 + (x - sh + 4);
+    for (int b = 0; b < sh; ++b) s8[b] = (char)(c >> (8 * (4 - sh + b)));
+}
+
 // v as %d prints it, nd = ndig(v) characters at o
 __device__ __forceinline__ void put_dec(char *o, int32_t v, int nd)
 {
     uint32_t x = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
     int e = nd;
-    do {
-        const uint32_t d = x / 10u;
-        o[--e] = (char)('0' + (x - 10u * d));
+    // two digits a step: r = x mod 100, its tens digit (r * 103) >> 10 (exact for r < 100)
+    while (x >= 100u) {
+        const uint32_t d = x / 100u, r = x - 100u * d, t = (r * 103u) >> 10;
+        o[--e] = (char)('0' + (r - 10u * t));
+        o[--e] = (char)('0' + t);
         x = d;
-    } while (x);
+    }
+    if (x >= 10u) {
+        const uint32_t t = (x * 103u) >> 10;
+        o[--e] = (char)('0' + (x - 10u * t));
+        o[--e] = (char)('0' + t);
+    } else {
+        o[--e] = (char)('0' + x);
+    }
     if (v < 0) o[0] = '-';
 }
 
 __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
 {
     __shared__ uint32_t s_text[RB / 64][QL][SLOT_TEXT];
-    __shared__ uint32_t s_ops[RB / 64][QL][WPL], s_rc[RB / 64][QL][WPL], s_gc[RB / 64][QL][WPL];
+    // transcript words staged after one zero word (fields below the staged ones read 0)
+    __shared__ uint32_t s_ops[RB / 64][QL][WPL + 1], s_rc[RB / 64][QL][WPL], s_gc[RB / 64][QL][WPL];
     __shared__ uint32_t s_name[RB / 64][QL][NAMEW];
     __shared__ char s_prefix[256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -406,6 +459,7 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
     const int sb = lane & ~(TPL - 1);              // the slot's first lane
     if (!a.names)
         for (int i = tid; i < a.prefix_len; i += RB) s_prefix[i] = a.prefix[i];
+    if (tid < (RB / 64) * QL) s_ops[tid / QL][tid % QL][0] = 0u;
     __syncthreads();
     const int nwaves = gridDim.x * (RB / 64);
     const int ntile = (a.n + 63) / 64;
@@ -431,14 +485,37 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
             off_l = a.off[tl];
             id_l = a.read_id ? a.read_id[tl] : tl;
         }
-        uint64_t todo = __ballot(len_l != 0 && grouped_line(len_l, inf.z, cw[4] >> 16));
-        while (todo) {
+        const bool mine = len_l != 0 && grouped_line(len_l, inf.z, cw[4] >> 16);
+        const uint64_t todo = __ballot(mine);
+        const int count = (int)__popcll(todo);
+        // the tile's lines in order of printed positions, so that the QL lines of a
+        // pass take about as many row steps and copy words each (a pass lasts as
+        // long as its longest line): a bitonic sort of (positions << 6 | lane) over
+        // the wave, the other candidates last
+        const uint32_t endp_l = (inf.z & 0xFFFFu) + (inf.z >> 16);
+        uint32_t order = ((mine ? (endp_l < 1022u ? endp_l : 1022u) : 1023u) << 6) | (uint32_t)lane;
+        if (count > QL) {
+            for (int k = 2; k <= 64; k <<= 1)
+                for (int jx = k >> 1; jx > 0; jx >>= 1) {
+                    const uint32_t o = (uint32_t)__shfl_xor((int)order, jx, 64);
+                    const bool keep_min = ((lane & jx) == 0) == ((lane & k) == 0);
+                    order = keep_min ? (o < order ? o : order) : (o > order ? o : order);
+                }
+        }
+        for (int base = 0; base < count; base += QL) {
             // the next QL lines: the owner lane of this lane's slot
             int owner = -1;
-            for (int g = 0; g < QL && todo; ++g) {
-                const int h = (int)__builtin_ctzll(todo);
-                todo &= todo - 1;
-                if (g == q) owner = h;
+            if (count > QL) {
+                const int si = base + q;
+                const uint32_t pick = from_lane(order, si);
+                owner = si < count ? (int)(pick & 63u) : -1;
+            } else {
+                uint64_t t = todo;   // one pass, in lane order
+                for (int g = 0; g < QL && t; ++g) {
+                    const int h = (int)__builtin_ctzll(t);
+                    t &= t - 1;
+                    if (g == q) owner = h;
+                }
             }
             const bool act = owner >= 0;
             const int src = act ? owner : 0;
@@ -465,7 +542,7 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
             const int glo = (int)((win_off + (uint32_t)end_row) >> 4);
             const int rnw = (int)((read_off + (uint32_t)min(end_col + endp, L)) >> 4) - rlo + 1;
             const int gnw = (int)((win_off + (uint32_t)min(end_row + endp, w)) >> 4) - glo + 1;
-            uint32_t *sops = s_ops[wv][q], *src_ = s_rc[wv][q], *sgc = s_gc[wv][q];
+            uint32_t *sops = s_ops[wv][q] + 1, *src_ = s_rc[wv][q], *sgc = s_gc[wv][q];
             // ---- stage
             if (act) {
                 if (endp > 0) {
@@ -502,17 +579,27 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                     name_n = a.prefix_len + ndi;
                 }
                 const int hdr = name_n + 1 + nds + 1 + ndt + 1 + 2 + 1;
-                // header: name \t start \t stop \t << or >> \t, every character but
-                // the digits (x counts from the end of the name)
-                for (int h = u; h < hdr; h += TPL) {
-                    const int x = h - name_n;
-                    if (x < 0) {
-                        if (a.names) line[h] = sname[h];
-                        else if (h < a.prefix_len) line[h] = s_prefix[h];
-                    } else if (x == 0 || x == 1 + nds || x == 2 + nds + ndt || x == 5 + nds + ndt) {
-                        line[h] = '\t';
-                    } else if (x == 3 + nds + ndt || x == 4 + nds + ndt) {
-                        line[h] = flag > 3 ? '<' : '>';
+                // header: name \t start \t stop \t << or >> \t: the name's characters
+                // (those of the prefix when names are prefix + id) over the lanes, the
+                // separators and direction characters a lane each, the digits below
+                const int nn = a.names ? name_n : a.prefix_len;
+                for (int h = u; h < nn; h += TPL) line[h] = a.names ? sname[h] : s_prefix[h];
+                {
+                    const char dir = flag > 3 ? '<' : '>';
+                    const int x = name_n + 3 + nds + ndt;   // the first direction character
+                    if (u == 4) {
+                        line[name_n] = '\t';
+                        line[hdr + rn] = '\t';                    // between the rows
+                    } else if (u == 5) {
+                        line[name_n + 1 + nds] = '\t';
+                        line[hdr + 2 * rn + 1] = '\t';            // before the score
+                    } else if (u == 6) {
+                        line[name_n + 2 + nds + ndt] = '\t';
+                        line[hdr + 2 * rn + 2 + ndsc] = '\n';
+                    } else if (u == 7) {
+                        line[x] = dir;
+                        line[x + 1] = dir;
+                        line[x + 2] = '\t';
                     }
                 }
                 // the digits: each of the four lanes prints its number as %d does
@@ -524,13 +611,6 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                     else if (u == 3) at = hdr + 2 * rn + 2;
                     if (at >= 0) put_dec(line + at, num, ndn);
                 }
-                if (u == 4) {
-                    line[hdr + rn] = '\t';                    // between the rows
-                    line[hdr + 2 * rn + 1] = '\t';            // before the score
-                    line[hdr + 2 * rn + 2 + ndsc] = '\n';
-                }
-                char *orow = line + hdr;
-                char *grow = orow + rn + 1;
                 // the line's positions split over its lanes in runs of a multiple of 4
                 // (at most CHUNK each)
                 const int chunk = (((endp + TPL - 1) / TPL) + 3) & ~3;
@@ -573,39 +653,56 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                     int rf = (int)(read_off + (uint32_t)col) - 16 * rlo;
                     int gf = (int)(win_off + (uint32_t)row) - 16 * glo;
                     const int jlo = 16 * wlo;
-                    // four positions at a time: their ops, then four characters per row
-                    // from one byte permute each ('_' for a gap)
+                    // slot bytes of position 0 in the two rows (place = position - lead)
+                    uint32_t *s32 = s_text[wv][q];
+                    const int xo = al + hdr - lead, xg = xo + rn + 1;
+                    uint32_t prev_r = 0u, prev_g = 0u;
+                    bool pfull = false;   // the group before was written whole but its tail
+                    // four positions at a time: their ops, then four characters per row;
+                    // a whole group goes into the slot as words (joined with the group
+                    // before at the slot's word boundaries), the first and last groups
+                    // of a run and the group that crosses lead as bytes
                     for (int i = 0; i < cnt; i += 4) {
                         // ops of positions p0 + i .. p0 + i + 3: fields j .. j + 3 with
                         // j = n - 4 - p0 - i (descending positions); fields below the
                         // staged words belong to positions past the line (taken as 0)
-                        const int j = n - 4 - p0 - i;
-                        const uint32_t og = j >= jlo ? fields4(sops, j - jlo)
-                                                     : fields4(sops, 0) << (2 * (jlo - j));
-                        const uint32_t rc8 = fields4(src_, rf), gc8 = fields4(sgc, gf);
-                        uint32_t rsel = 0, gsel = 0;
-                        int ro = 0, go = 0;
+                        const int j = n - 4 - p0 - i;   // j - jlo >= -3
+                        const uint32_t og = fields4(sops, j - jlo);
+                        // op of position p0 + i + t in byte t
+                        const uint32_t ob = __builtin_amdgcn_perm(0u, spread4(og), 0x00010203u);
+                        const uint32_t lo = ob & 0x01010101u, hi = (ob >> 1) & 0x01010101u;
+                        const uint32_t tops = lo & ~hi, lefts = hi & ~lo;   // TOP 1, LEFT 2
+                        const uint32_t rch = row_chars4(fields4(src_, rf), tops);
+                        const uint32_t gch = row_chars4(fields4(sgc, gf), lefts);
+                        rf += 4 - (int)__builtin_popcount(tops);
+                        gf += 4 - (int)__builtin_popcount(lefts);
+                        const int P = p0 + i;
+                        if (i + 4 <= cnt && P >= lead) {
+                            put_head(s32, xo + P, rch, prev_r, pfull);
+                            put_head(s32, xg + P, gch, prev_g, pfull);
+                            prev_r = rch;
+                            prev_g = gch;
+                            pfull = true;
+                        } else {
+                            if (pfull) {
+                                put_tail(s32, xo + P - 4, prev_r);
+                                put_tail(s32, xg + P - 4, prev_g);
+                            }
+                            pfull = false;
+                            char *s8 = (char *)s32;
 #pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const uint32_t op = (og >> (6 - 2 * t)) & 3u;   // position p0 + i + t
-                            const bool top = op == SRM_OP_TOP, left = op == SRM_OP_LEFT;
-                            rsel |= (top ? 4u : (rc8 >> (2 * ro)) & 3u) << (8 * t);
-                            gsel |= (left ? 4u : (gc8 >> (2 * go)) & 3u) << (8 * t);
-                            ro += top ? 0 : 1;
-                            go += left ? 0 : 1;
-                        }
-                        const uint32_t rch = __builtin_amdgcn_perm(0x5F5F5F5Fu, 0x54474341u, rsel);
-                        const uint32_t gch = __builtin_amdgcn_perm(0x5F5F5F5Fu, 0x54474341u, gsel);
-                        const int pl = p0 + i - lead;          // row place of position p0 + i
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            if (i + t < cnt && pl + t >= 0) {
-                                orow[pl + t] = (char)(rch >> (8 * t));
-                                grow[pl + t] = (char)(gch >> (8 * t));
+                            for (int t = 0; t < 4; ++t) {
+                                if (i + t < cnt && P + t >= lead) {
+                                    s8[xo + P + t] = (char)(rch >> (8 * t));
+                                    s8[xg + P + t] = (char)(gch >> (8 * t));
+                                }
                             }
                         }
-                        rf += ro;
-                        gf += go;
+                    }
+                    if (pfull) {
+                        const int P = p0 + ((cnt - 1) & ~3);   // the last group
+                        put_tail(s32, xo + P, prev_r);
+                        put_tail(s32, xg + P, prev_g);
                     }
                 }
             }
@@ -616,18 +713,20 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
             if (act) {
                 uint32_t *o32 = (uint32_t *)(a.out + (loff - (uint64_t)al));
                 const int end = al + (int)llen;
-                const int nd = (end + 3) >> 2;
+                const int last = end >> 2;   // the word of the line's end
                 const uint32_t *t32 = s_text[wv][q];
-                for (int i = u; i < nd; i += TPL) {
-                    const uint32_t v = t32[i];
-                    const int b0 = 4 * i;
-                    if (b0 >= al && b0 + 4 <= end) {
-                        o32[i] = v;
-                    } else {
-                        char *o8 = (char *)(o32 + i);
-                        for (int b = 0; b < 4; ++b)
-                            if (b0 + b >= al && b0 + b < end) o8[b] = (char)(v >> (8 * b));
-                    }
+                // whole words [al ? 1 : 0, last)
+                for (int i = (al ? 1 : 0) + u; i < last; i += TPL) o32[i] = t32[i];
+                // the first and last words, shared with the neighbouring lines: bytes
+                // [al, 4) of word 0 when al > 0, bytes of word last below end
+                const bool first = u == 0 && al;
+                const bool tail = u == 1 && (end & 3) && !(al && last == 0);
+                if (first || tail) {
+                    const int w = first ? 0 : last;
+                    const int b0 = first ? al : 0, b1 = first ? (end < 4 ? end : 4) : (end & 3);
+                    const uint32_t v = t32[w];
+                    char *o8 = (char *)(o32 + w);
+                    for (int b = b0; b < b1; ++b) o8[b] = (char)(v >> (8 * b));
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
