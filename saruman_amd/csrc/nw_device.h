@@ -72,7 +72,7 @@ struct RecArgs {
     int prefix_len;
     uint64_t *len;                  // [n + 1] line lengths (0: no record)
     uint4 *info;                    // [n] start, stop, lead | chars << 16
-    uint32_t *hits;                 // records counted
+    uint32_t *hits;                 // [2] records counted, lines longer than rec_write_lines takes
     const uint64_t *off;            // [n + 1] line offsets
     char *out;
 };

@@ -372,7 +372,7 @@ int run_launches(srm_job *j, int kernel, bool timed)
         // records of every hit into the output sized at set-up: counter reset (a
         // kernel, so that a graph can hold it), line lengths, their scan, the text
         srm::RecArgs a = rec_args(j, true);
-        HIP_TRY(srm::launch_reset(a.hits, 1, st));
+        HIP_TRY(srm::launch_reset(a.hits, 2, st));
         HIP_TRY(srm::launch_rec_measure(a, st));
         size_t tmp = j->rec_tmp;
         HIP_TRY(srm::rec_scan(j->d_rec_tmp.p, tmp, a.len, (uint64_t *)j->d_rec_off.p, j->n, st));
@@ -1215,7 +1215,7 @@ long long srm_job_records(srm_job *j, const int64_t *pos, const int32_t *read_id
     srm_ctx *ctx = j->ctx;
     const int n = j->n;
     srm::RecArgs a = rec_args(j, true);
-    HIP_TRY(hipMemsetAsync(a.hits, 0, 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(a.hits, 0, 8, ctx->stream));
     HIP_TRY(srm::launch_rec_measure(a, ctx->stream));
     size_t tmp = j->rec_tmp;
     HIP_TRY(srm::rec_scan(j->d_rec_tmp.p, tmp, a.len, (uint64_t *)j->d_rec_off.p, n, ctx->stream));

@@ -43,9 +43,25 @@ __device__ __forceinline__ int name_len(const RecArgs &a, int32_t id)
     return a.prefix_len + ndig(id);
 }
 
+// line sizes of rec_write_lines (below), which writes the lines grouped_line() accepts
+constexpr int QL = 8;                // lines per wave
+constexpr int TPL = 8;               // lanes per line
+constexpr int CHUNK = 40;            // printed positions per lane at most
+constexpr int MAXP = TPL * CHUNK;    // printed positions per line (320)
+constexpr int LCAP = 704;            // text bytes per line
+constexpr int WPL = MAXP / 16 + 3;   // staged words per line: transcript, read codes, window codes
+constexpr int NAMEW = 64;            // staged name words per line (name_stride <= 256)
+
+__device__ __forceinline__ bool grouped_line(uint64_t len, uint32_t lead_rn, uint32_t bits)
+{
+    const int endp = (int)(lead_rn & 0xFFFFu) + (int)(lead_rn >> 16);
+    return len > 0 && len <= (uint64_t)(LCAP - 4) && endp <= MAXP && !(bits & SRM_CAND_BYTES);
+}
+
 // leading gaps, closing gaps, indel scan and printed length of one hit
 // (cuda-functions.cu:866-980), a transcript word at a time; returns the line length
-__device__ uint64_t measure_one(const RecArgs &a, int t)
+// and whether rec_write_lines writes the line (else rec_write does)
+__device__ uint64_t measure_one(const RecArgs &a, int t, bool &grouped)
 {
     const srm_cand c = a.cand[t];
     const srm_result r = a.res[t];
@@ -62,50 +78,50 @@ __device__ uint64_t measure_one(const RecArgs &a, int t)
     const int32_t stop = (int32_t)((uint32_t)(int32_t)chars - (uint32_t)ins + start - 1u);
     a.info[t] = make_uint4(start, (uint32_t)stop, (uint32_t)lead | ((uint32_t)rn << 16), 0u);
     const int32_t id = a.read_id ? a.read_id[t] : t;
-    return (uint64_t)(name_len(a, id) + ndig((int32_t)start) + ndig(stop) + 2 + 2 * rn +
-                      ndig((int)r.score) + 7);
+    const uint64_t len = (uint64_t)(name_len(a, id) + ndig((int32_t)start) + ndig(stop) + 2 +
+                                    2 * rn + ndig((int)r.score) + 7);
+    grouped = grouped_line(len, (uint32_t)lead | ((uint32_t)rn << 16), c.bits);
+    return len;
 }
 
-// a grid that strides over the candidates; the hits are summed per block and
-// added to the counter with one atomic per block (the counter is one word that
-// every block would otherwise update once per wave)
+// a grid that strides over the candidates; the hits (hits[0]) and the lines left
+// to rec_write (hits[1]) are summed per block and added with one atomic each per
+// block (the counters are words that every block would otherwise update once per
+// wave)
 __global__ void __launch_bounds__(RB) rec_measure(RecArgs a)
 {
-    __shared__ uint32_t s_hits[RB / 64];
-    uint32_t hits = 0;
+    __shared__ uint32_t s_hits[RB / 64][2];
+    uint32_t hits = 0, longs = 0;
     for (int t = blockIdx.x * RB + threadIdx.x; t <= a.n; t += gridDim.x * RB) {
         if (t == a.n) {
             a.len[t] = 0;                           // the scan's tail element
             continue;
         }
         const bool hit = a.res[t].score <= (int)a.cand[t].k;
-        a.len[t] = hit ? measure_one(a, t) : 0;
+        bool grouped = true;
+        a.len[t] = hit ? measure_one(a, t, grouped) : 0;
         hits += hit;
+        longs += hit && !grouped;
     }
-    // block sum: waves by reduction in registers, then the block's waves in LDS
-    for (int d = 32; d >= 1; d >>= 1) hits += (uint32_t)__shfl_xor((int)hits, d, 64);
-    if ((threadIdx.x & 63) == 0) s_hits[threadIdx.x >> 6] = hits;
+    // block sums: waves by reduction in registers, then the block's waves in LDS
+    for (int d = 32; d >= 1; d >>= 1) {
+        hits += (uint32_t)__shfl_xor((int)hits, d, 64);
+        longs += (uint32_t)__shfl_xor((int)longs, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_hits[threadIdx.x >> 6][0] = hits;
+        s_hits[threadIdx.x >> 6][1] = longs;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t b = 0;
-        for (int w = 0; w < RB / 64; ++w) b += s_hits[w];
+        uint32_t b = 0, l = 0;
+        for (int w = 0; w < RB / 64; ++w) {
+            b += s_hits[w][0];
+            l += s_hits[w][1];
+        }
         if (b) atomicAdd(a.hits, b);
+        if (l) atomicAdd(a.hits + 1, l);
     }
-}
-
-// line sizes of rec_write_lines (below), which writes the lines grouped_line() accepts
-constexpr int QL = 8;                // lines per wave
-constexpr int TPL = 8;               // lanes per line
-constexpr int CHUNK = 40;            // printed positions per lane at most
-constexpr int MAXP = TPL * CHUNK;    // printed positions per line (320)
-constexpr int LCAP = 704;            // text bytes per line
-constexpr int WPL = MAXP / 16 + 3;   // staged words per line: transcript, read codes, window codes
-constexpr int NAMEW = 64;            // staged name words per line (name_stride <= 256)
-
-__device__ __forceinline__ bool grouped_line(uint64_t len, uint32_t lead_rn, uint32_t bits)
-{
-    const int endp = (int)(lead_rn & 0xFFFFu) + (int)(lead_rn >> 16);
-    return len > 0 && len <= (uint64_t)(LCAP - 4) && endp <= MAXP && !(bits & SRM_CAND_BYTES);
 }
 
 // ---------------------------------------------------------------------------
@@ -166,6 +182,7 @@ __device__ __forceinline__ uint32_t lane_u32(uint32_t v, int j)
 __global__ void __launch_bounds__(RB) rec_write(RecArgs a)
 {
     __shared__ uint32_t lds[RB / 64][LINE_LDS / 4];
+    if (a.hits[1] == 0) return;   // no line of this kind (counted by rec_measure)
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     char *buf = (char *)lds[wv];
