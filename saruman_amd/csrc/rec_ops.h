@@ -37,6 +37,16 @@ SRM_RHD inline uint32_t lefts_of(uint32_t x) { return (x >> 1) & ~x & 0x55555555
 SRM_RHD inline uint32_t diags_of(uint32_t x) { return ~(x | (x >> 1)) & 0x55555555u; }
 SRM_RHD inline uint32_t non_tops_of(uint32_t x) { return (~x | (x >> 1)) & 0x55555555u; }
 
+// bit index of the m-th set bit of s counted from the top (1 <= m <= popc(s)): the
+// largest b with popc(s >> b) >= m, by a binary search over b
+SRM_RHD inline int nth_bit_from_top(uint32_t s, int m)
+{
+    int b = 0;
+    for (int step = 16; step >= 1; step >>= 1)
+        if (popc32(s >> (b + step)) >= m) b += step;
+    return b;
+}
+
 struct Measure {
     int lead = 0;    // leading read-row gaps (cuda-functions.cu:878-886)
     int trail = 0;   // closing read-row gaps (:897-907)
@@ -48,54 +58,37 @@ struct Measure {
 // The scans of one hit's transcript (n ops, window length w).  The indel scan
 // visits positions lead, lead + 1, .. while i < w - trail - lead + z (i positions
 // done, z of them gaps), that is while fewer than w - trail - lead DIAG ops have
-// been seen, and stops with eor = i at the row's end.  ld(W) gives word W; the
-// words are asked for eight at a time, so that a device thread waits for memory
-// once per eight words rather than once per word.
+// been seen, and stops with eor = i at the row's end.  ld(W) gives word W.  The
+// gap runs at the two ends are read a word at a time (they end in their first
+// word unless it is all TOP ops); the indel scan asks for its words eight at a
+// time, so that a device thread waits for memory once per eight words rather than
+// once per word.
 template <class Load>
 SRM_RHD inline Measure measure_ops_by(Load ld, int n, int w)
 {
     constexpr int B = 8;
     Measure m;
     if (n <= 0) return m;   // the scan, if any, meets the NUL at once: eor 0
-    const int wtop = (n - 1) >> 4;
+    const int wtop = (n - 1) >> 4, ftop = (n - 1) & 15;
     uint32_t x[B];
     // leading TOP ops from op n - 1 downwards
-    bool done = false;
-    for (int base = wtop; base >= 0 && !done; base -= B) {
-#pragma unroll
-        for (int k = 0; k < B; ++k) x[k] = base - k >= 0 ? ld(base - k) : 0u;
-#pragma unroll
-        for (int k = 0; k < B; ++k) {
-            const int W = base - k;
-            if (done || W < 0) continue;
-            const int f_hi = W == wtop ? (n - 1) & 15 : 15;
-            const uint32_t nt = non_tops_of(x[k]) & field_mask(0, f_hi);
-            if (!nt) {
-                m.lead += f_hi + 1;
-            } else {
-                m.lead += f_hi - (top_bit(nt) >> 1);
-                done = true;
-            }
+    for (int W = wtop, f_hi = ftop; W >= 0; --W, f_hi = 15) {
+        const uint32_t nt = non_tops_of(ld(W)) & field_mask(0, f_hi);
+        if (nt) {
+            m.lead += f_hi - (top_bit(nt) >> 1);
+            break;
         }
+        m.lead += f_hi + 1;
     }
     // closing TOP ops from op 0 upwards
-    done = false;
-    for (int base = 0; base <= wtop && !done; base += B) {
-#pragma unroll
-        for (int k = 0; k < B; ++k) x[k] = base + k <= wtop ? ld(base + k) : 0u;
-#pragma unroll
-        for (int k = 0; k < B; ++k) {
-            const int W = base + k;
-            if (done || W > wtop) continue;
-            const int f_hi = W == wtop ? (n - 1) & 15 : 15;
-            const uint32_t nt = non_tops_of(x[k]) & field_mask(0, f_hi);
-            if (!nt) {
-                m.trail += f_hi + 1;
-            } else {
-                m.trail += __builtin_ctz(nt) >> 1;
-                done = true;
-            }
+    for (int W = 0; W <= wtop; ++W) {
+        const int f_hi = W == wtop ? ftop : 15;
+        const uint32_t nt = non_tops_of(ld(W)) & field_mask(0, f_hi);
+        if (nt) {
+            m.trail += __builtin_ctz(nt) >> 1;
+            break;
         }
+        m.trail += f_hi + 1;
     }
     const int need = w - m.trail - m.lead;   // DIAG ops the scan waits for
     const int jtop = n - 1 - m.lead;         // op of position lead
@@ -119,9 +112,7 @@ SRM_RHD inline Measure measure_ops_by(Load ld, int n, int w)
                 continue;
             }
             // the (need - diag)-th DIAG field from the top ends the scan
-            uint32_t s = d;
-            for (int t = need - diag; t > 1; --t) s &= ~(0x80000000u >> __builtin_clz(s));
-            const uint32_t range = field_mask(top_bit(s) >> 1, f_hi);
+            const uint32_t range = field_mask(nth_bit_from_top(d, need - diag) >> 1, f_hi);
             m.del += popc32(tops_of(x[k]) & range);
             m.ins += popc32(lefts_of(x[k]) & range);
             return m;
