@@ -464,17 +464,20 @@ __device__ __forceinline__ void put_dec(char *o, int32_t v, int nd)
     if (v < 0) o[0] = '-';
 }
 
+template <bool NAMES>   // names from a table (else prefix + id)
 __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
 {
     __shared__ uint32_t s_text[RB / 64][QL][SLOT_TEXT];
     // transcript words staged after one zero word (fields below the staged ones read 0)
     __shared__ uint32_t s_ops[RB / 64][QL][WPL + 1], s_rc[RB / 64][QL][WPL], s_gc[RB / 64][QL][WPL];
-    __shared__ uint32_t s_name[RB / 64][QL][NAMEW];
+    // a name table's names staged per line (not needed for prefix + id: the smaller
+    // block leaves room for a fifth block on a CU)
+    __shared__ uint32_t s_name[NAMES ? RB / 64 : 1][NAMES ? QL : 1][NAMES ? NAMEW : 1];
     __shared__ char s_prefix[256];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int q = lane / TPL, u = lane % TPL;      // this lane's line slot and place in it
     const int sb = lane & ~(TPL - 1);              // the slot's first lane
-    if (!a.names)
+    if (!NAMES)
         for (int i = tid; i < a.prefix_len; i += RB) s_prefix[i] = a.prefix[i];
     if (tid < (RB / 64) * QL) s_ops[tid / QL][tid % QL][0] = 0u;
     __syncthreads();
@@ -568,9 +571,9 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                     for (int k = u; k < rnw; k += TPL) src_[k] = a.read_words[rlo + k];
                     for (int k = u; k < gnw; k += TPL) sgc[k] = a.win_words[glo + k];
                 }
-                if (a.names) {
+                if (NAMES) {
                     const char *nm = a.names + (int64_t)lid * a.name_stride;
-                    char *sn = (char *)s_name[wv][q];
+                    char *sn = (char *)s_name[NAMES ? wv : 0][NAMES ? q : 0];
                     for (int i = u; i < a.name_stride; i += TPL) sn[i] = nm[i];
                 }
             }
@@ -587,9 +590,9 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                 const int ndn = ndig(num);
                 const int nds = __shfl(ndn, sb, 64), ndt = __shfl(ndn, sb + 1, 64);
                 const int ndi = __shfl(ndn, sb + 2, 64), ndsc = __shfl(ndn, sb + 3, 64);
-                const char *sname = (const char *)s_name[wv][q];
+                const char *sname = (const char *)s_name[NAMES ? wv : 0][NAMES ? q : 0];
                 int name_n;
-                if (a.names) {
+                if (NAMES) {
                     name_n = 0;
                     while (name_n < a.name_stride - 1 && sname[name_n]) ++name_n;
                 } else {
@@ -599,8 +602,8 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                 // header: name \t start \t stop \t << or >> \t: the name's characters
                 // (those of the prefix when names are prefix + id) over the lanes, the
                 // separators and direction characters a lane each, the digits below
-                const int nn = a.names ? name_n : a.prefix_len;
-                for (int h = u; h < nn; h += TPL) line[h] = a.names ? sname[h] : s_prefix[h];
+                const int nn = NAMES ? name_n : a.prefix_len;
+                for (int h = u; h < nn; h += TPL) line[h] = NAMES ? sname[h] : s_prefix[h];
                 {
                     const char dir = flag > 3 ? '<' : '>';
                     const int x = name_n + 3 + nds + ndt;   // the first direction character
@@ -624,7 +627,7 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                     int at = -1;
                     if (u == 0) at = name_n + 1;
                     else if (u == 1) at = name_n + 2 + nds;
-                    else if (u == 2 && !a.names) at = a.prefix_len;
+                    else if (u == 2 && !NAMES) at = a.prefix_len;
                     else if (u == 3) at = hdr + 2 * rn + 2;
                     if (at >= 0) put_dec(line + at, num, ndn);
                 }
@@ -770,7 +773,12 @@ hipError_t launch_rec_write(const RecArgs &a, hipStream_t st)
     // a time, then the others one at a time
     const int tiles = (a.n + 63) / 64;
     const int waves = tiles < 65536 * 4 ? tiles : 65536 * 4;
-    hipLaunchKernelGGL(rec_write_lines, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
+    if (a.names)
+        hipLaunchKernelGGL(rec_write_lines<true>, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL(rec_write_lines<false>, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB),
+                           0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rec_write, dim3((waves + RB / 64 - 1) / (RB / 64)), dim3(RB), 0, st, a);
