@@ -54,8 +54,8 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 PMC_PROFILE = [
     {"workload": "mixed", "candidates": 1774238,
      "tuning": {"blocks_per_cu": [0, 0, 0, 0, 0, 0, 0, 0, 0], "merge_below": 2, "class_streams": 0, "tile_cands": 64, "block_waves": 1, "split_len": 0, "walk_layout": 1},
-     "kernel_sha": "b2e0e15e9101", "band_bytes": 1517920082, "band_valu_busy": 0.552,
-     "records_bytes": 1327580715},
+     "kernel_sha": "92d8c61c78a5", "band_bytes": 1515447425, "band_valu_busy": 0.566,
+     "records_bytes": 1498005480},
 ]
 
 
@@ -119,8 +119,8 @@ def band_cells(L, k, flag):
 
 def hbm_bytes(L, w, k):
     """Algorithmic HBM bytes per candidate of the band phase: 2-bit read and window,
-    16 B descriptor, 2-bit transcript, 8 B result."""
-    return (np.ceil(L / 4) + np.ceil(w / 4) + 16 + np.ceil(2 * (L + 3 * k) / 8) + 8).astype(
+    20 B descriptor (srm_cand), 2-bit transcript, 8 B result."""
+    return (np.ceil(L / 4) + np.ceil(w / 4) + 20 + np.ceil(2 * (L + 3 * k) / 8) + 8).astype(
         np.float64)
 
 
