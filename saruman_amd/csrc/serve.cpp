@@ -50,6 +50,49 @@ inline char comp(char c)
     return t[(uint8_t)c];
 }
 
+// host threads for a request's per-byte passes: one below 1 MB of work, else up
+// to 8 (each slot and the filter stage run their own)
+unsigned host_threads(size_t bytes)
+{
+    if (bytes < ((size_t)1 << 20)) return 1;
+    return std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+}
+
+// fn(u, lo, hi) on T contiguous ranges of [0, n), range u on thread u
+template <class Fn>
+void par_ranges(unsigned T, size_t n, Fn fn)
+{
+    if (T <= 1 || n < T) {
+        fn(0u, (size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (unsigned u = 0; u + 1 < T; ++u) th.emplace_back(fn, u, n * u / T, n * (u + 1) / T);
+    fn(T - 1, n * (T - 1) / T, n);
+    for (auto &x : th) x.join();
+}
+
+// srm_pack_codes over ranges of whole code words on several threads (a word is
+// packed by one thread: the ranges start where code_off + start is a multiple of 16)
+void pack_codes_par(const char *seq, size_t n, uint32_t *words, size_t code_off)
+{
+    const unsigned T = host_threads(n);
+    if (T <= 1) {
+        srm_pack_codes(seq, n, words, code_off);
+        return;
+    }
+    auto cut = [&](size_t x) -> size_t {   // the first word boundary at or after x
+        if (x == 0 || x >= n) return x >= n ? n : 0;
+        const size_t p = code_off + x;
+        return std::min(n, x + ((16 - (p & 15)) & 15));
+    };
+    par_ranges(T, n, [&](unsigned, size_t lo, size_t hi) {
+        const size_t a = cut(lo), b = cut(hi);
+        if (b > a) srm_pack_codes(seq + a, b - a, words, code_off + a);
+    });
+}
+
 struct Request {
     uint64_t id = 0;
     size_t max_len = 0;   // longest read
@@ -93,6 +136,8 @@ struct srm_server {
     srm_dfilter *dfilter = nullptr;   // set when the filter runs on the device
     std::vector<srm_ctx *> ctx;       // one per slot; slot 0 owns the resident genome
     std::vector<srm_job *> job;
+    std::vector<char *> stage;        // per slot: page-locked buffer for the record text
+    std::vector<size_t> stage_cap;
     Clock::time_point t0;
 
     std::mutex mu;
@@ -138,11 +183,14 @@ void srm_server::prepare(Request &r)
     const size_t n = r.len.size();
     const size_t total = r.reads.size();
     r.reads_rev.resize(total);
-    for (size_t i = 0; i < n; ++i) {
-        const char *src = r.reads.data() + r.off[i];
-        char *dst = &r.reads_rev[r.off[i]];
-        for (int j = 0; j < r.len[i]; ++j) dst[j] = comp(src[r.len[i] - 1 - j]);
-    }
+    // reverse complements, reads split over the threads
+    par_ranges(host_threads(total), n, [&](unsigned, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const char *src = r.reads.data() + r.off[i];
+            char *dst = &r.reads_rev[r.off[i]];
+            for (int j = 0; j < r.len[i]; ++j) dst[j] = comp(src[r.len[i] - 1 - j]);
+        }
+    });
     // both strands through the reference filter semantics (the device filter runs
     // both over one upload of the reads)
     srm_filter *both[2] = {nullptr, nullptr};
@@ -198,8 +246,8 @@ void srm_server::prepare(Request &r)
     }
     // read codes: forward reads, then their reverse complements
     r.read_words.assign((2 * total + 2 * SRM_CODE_PAD + 15) / 16 + 1, 0u);
-    srm_pack_codes(r.reads.data(), total, r.read_words.data(), SRM_CODE_PAD);
-    srm_pack_codes(r.reads_rev.data(), total, r.read_words.data(), SRM_CODE_PAD + total);
+    pack_codes_par(r.reads.data(), total, r.read_words.data(), SRM_CODE_PAD);
+    pack_codes_par(r.reads_rev.data(), total, r.read_words.data(), SRM_CODE_PAD + total);
     size_t maxL = 0;
     for (auto L : r.len) maxL = std::max<size_t>(maxL, L);
     r.max_len = maxL;
@@ -241,43 +289,60 @@ void srm_server::run_on_slot(int s, Request &r)
     std::vector<char> mapped(r.len.size(), 0);
     {
         // perfect hits: "r<id>\t<start>\t<stop>\t<strand>\t<genome>\t<read>\t0\n", as
-        // the reference prints them (%d fields), written directly
+        // the reference prints them (%d fields), written directly; the hits of both
+        // strands in order, split over threads that each write their own part
+        const size_t n0 = r.perf_read[0].size(), np = n0 + r.perf_read[1].size();
         size_t bytes = 0;
         for (int sd = 0; sd < 2; ++sd)
             for (int32_t i : r.perf_read[sd]) bytes += 2 * (size_t)r.len[(size_t)i] + 48;
-        out.reserve(bytes);
-        auto put_int = [&](long long v) {
-            char b[24];
-            int q = 0;
-            const bool neg = v < 0;
-            unsigned long long u = neg ? 0ull - (unsigned long long)v : (unsigned long long)v;
-            do {
-                b[q++] = (char)('0' + u % 10);
-                u /= 10;
-            } while (u);
-            if (neg) out.push_back('-');
-            while (q) out.push_back(b[--q]);
-        };
-        for (int sd = 0; sd < 2; ++sd) {
-            for (size_t p = 0; p < r.perf_read[sd].size(); ++p) {
+        const unsigned T = host_threads(bytes);
+        std::vector<std::string> part(T);
+        par_ranges(T, np, [&](unsigned u, size_t lo, size_t hi) {
+            std::string &o = part[u];
+            size_t b = 0;
+            for (size_t x = lo; x < hi; ++x) {
+                const int sd = x < n0 ? 0 : 1;
+                b += 2 * (size_t)r.len[(size_t)r.perf_read[sd][sd ? x - n0 : x]] + 48;
+            }
+            o.reserve(b);
+            auto put_int = [&](long long v) {
+                char d[24];
+                int q = 0;
+                const bool neg = v < 0;
+                unsigned long long w = neg ? 0ull - (unsigned long long)v : (unsigned long long)v;
+                do {
+                    d[q++] = (char)('0' + w % 10);
+                    w /= 10;
+                } while (w);
+                if (neg) o.push_back('-');
+                while (q) o.push_back(d[--q]);
+            };
+            for (size_t x = lo; x < hi; ++x) {
+                const int sd = x < n0 ? 0 : 1;
+                const size_t p = sd ? x - n0 : x;
                 const int32_t i = r.perf_read[sd][p];
                 const int64_t S = r.perf_pos[sd][p];
                 const int L = r.len[(size_t)i];
                 const char *rd = (sd ? r.reads_rev.data() : r.reads.data()) + r.off[(size_t)i];
-                out.push_back('r');
+                o.push_back('r');
                 put_int(i);
-                out.push_back('\t');
+                o.push_back('\t');
                 put_int((int)S);
-                out.push_back('\t');
+                o.push_back('\t');
                 put_int((int)(S + L - 1));
-                out.append(sd ? "\t<<\t" : "\t>>\t");
-                out.append(genome.data() + S, strnlen(genome.data() + S, (size_t)L));   // %.*s
-                out.push_back('\t');
-                out.append(rd, strnlen(rd, (size_t)L));
-                out.append("\t0\n");
-                mapped[(size_t)i] = 1;
+                o.append(sd ? "\t<<\t" : "\t>>\t");
+                o.append(genome.data() + S, strnlen(genome.data() + S, (size_t)L));   // %.*s
+                o.push_back('\t');
+                o.append(rd, strnlen(rd, (size_t)L));
+                o.append("\t0\n");
             }
-        }
+        });
+        size_t tot = 0;
+        for (auto &x : part) tot += x.size();
+        out.reserve(tot);
+        for (auto &x : part) out.append(x);
+        for (int sd = 0; sd < 2; ++sd)
+            for (int32_t i : r.perf_read[sd]) mapped[(size_t)i] = 1;
     }
     r.st.n_perfect = (long long)(r.perf_read[0].size() + r.perf_read[1].size());
     if (n > 0 && cfg.gpu_records) {
@@ -287,9 +352,31 @@ void srm_server::run_on_slot(int s, Request &r)
         long long total = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r",
                                           nullptr, 0, &hits);
         if (total > 0) {
-            r.rec_text.reset(new char[(size_t)total]);
-            total = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r",
-                                    r.rec_text.get(), (size_t)total, &hits);
+            // the text comes back into the slot's page-locked buffer (a fast copy),
+            // then goes to the request's own buffer on several threads
+            const size_t want = (size_t)total;
+            if (stage_cap[(size_t)s] < want) {
+                srm_host_free(stage[(size_t)s]);
+                const size_t cap = want + want / 4 + 4096;
+                stage[(size_t)s] = (char *)srm_host_alloc(cap);
+                stage_cap[(size_t)s] = stage[(size_t)s] ? cap : 0;
+            }
+            char *dst = stage[(size_t)s];
+            std::unique_ptr<char[]> direct;
+            if (!dst) {   // no page-locked memory: straight into the request's buffer
+                direct.reset(new char[want]);
+                dst = direct.get();
+            }
+            total = srm_job_records(j, r.pos.data(), r.rid.data(), nullptr, 0, 0, "r", dst, want,
+                                    &hits);
+            if (total > 0 && direct) {
+                r.rec_text = std::move(direct);
+            } else if (total > 0) {
+                r.rec_text.reset(new char[(size_t)total]);
+                char *to = r.rec_text.get();
+                par_ranges(host_threads((size_t)total), (size_t)total,
+                           [&](unsigned, size_t lo, size_t hi) { std::memcpy(to + lo, dst + lo, hi - lo); });
+            }
         }
         if (total < 0) {
             set_failed(r.st, (int)total, srm_last_error());
@@ -504,6 +591,8 @@ int srm_server_create(const srm_server_config *cfg, const char *genome, size_t g
         S->job.push_back(j);
     }
     S->slot_req.assign((size_t)cfg->n_slots, nullptr);
+    S->stage.assign((size_t)cfg->n_slots, nullptr);
+    S->stage_cap.assign((size_t)cfg->n_slots, 0);
     S->slot_busy.assign((size_t)cfg->n_slots, false);
     S->reserved.assign((size_t)cfg->n_slots, 0);
     for (srm_ctx *c : S->ctx) S->base.push_back(srm_ctx_pool_used(c));
@@ -631,6 +720,7 @@ void srm_server_destroy(srm_server *S)
         srm_job_destroy(S->job[i]);
         srm_ctx_destroy(S->ctx[i]);
     }
+    for (char *b : S->stage) srm_host_free(b);
     srm_dfilter_destroy(S->dfilter);
     if (S->fctx) srm_ctx_destroy(S->fctx);
     srm_index_destroy(S->index);
