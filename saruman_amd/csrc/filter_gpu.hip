@@ -29,6 +29,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "filter_internal.h"
@@ -330,6 +331,12 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
     a.list_base = G->list_base.as<uint32_t>();
     a.list_cnt = G->list_cnt.as<uint32_t>();
     a.list_start = G->list_start.as<uint32_t>();
+    // per strand: the device's entries and perfect hits for the batch pass, which
+    // runs on the host once both strands' device work is done (the two strands'
+    // passes side by side)
+    std::vector<uint32_t> meta_s[2];
+    std::vector<int32_t> pos_s[2], perf_s[2];
+    std::vector<int8_t> flag_s[2];
     for (int dr : {0, 4}) {
         if (!(strands & (dr ? 2 : 1))) continue;
         a.dr = dr;
@@ -354,9 +361,9 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
         hipLaunchKernelGGL(fl_fill, dim3(grid), dim3(FB), 0, G->st, a);
         FL_TRY(hipGetLastError());
 
-        std::vector<uint32_t> meta;
-        std::vector<int32_t> pos, perf;
-        std::vector<int8_t> flag;
+        std::vector<uint32_t> &meta = meta_s[dr ? 1 : 0];
+        std::vector<int32_t> &pos = pos_s[dr ? 1 : 0], &perf = perf_s[dr ? 1 : 0];
+        std::vector<int8_t> &flag = flag_s[dr ? 1 : 0];
         FL_RC(G->pair(a, false, nullptr, (uint32_t)n_reads, ECAP, PCAP, meta, pos, flag, perf));
         // reads that outgrew the LDS slots run again with slots in HBM
         std::vector<uint32_t> over;
@@ -397,9 +404,18 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
         FL_TRY(hipEventRecord(G->ev[1], G->st));
         FL_TRY(hipEventSynchronize(G->ev[1]));
         FL_TRY(hipEventElapsedTime(&G->last_ms, G->ev[0], G->ev[1]));
-        srm_filter *F = batch_pass(MetaAcc{meta, pos, flag, perf}, n_reads, read_len, read_k,
-                                   G->glen, blocked, chunk_value);
-        *(dr ? out_rev : out_fwd) = F;
+    }
+    auto pass = [&](int x) {
+        *(x ? out_rev : out_fwd) =
+            batch_pass(MetaAcc{meta_s[x], pos_s[x], flag_s[x], perf_s[x]}, n_reads, read_len,
+                       read_k, G->glen, blocked, chunk_value);
+    };
+    if ((strands & 3) == 3) {
+        std::thread other(pass, 1);
+        pass(0);
+        other.join();
+    } else {
+        pass((strands & 1) ? 0 : 1);
     }
     return SRM_OK;
 }

@@ -404,10 +404,10 @@ hipError_t launch_gather_cand(const srm_cand *cand, const uint32_t *perm, srm_ca
     return hipGetLastError();
 }
 
-// host threads for a planning pass over n items: one below 200k, else up to 8
+// host threads for a planning pass over n items: one below 64k, else up to 8
 unsigned plan_threads(size_t n)
 {
-    if (n < 200000) return 1;
+    if (n < 65536) return 1;
     return std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
 }
 
