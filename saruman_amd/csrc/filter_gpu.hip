@@ -411,9 +411,14 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
                        read_k, G->glen, blocked, chunk_value);
     };
     if ((strands & 3) == 3) {
-        std::thread other(pass, 1);
+        std::thread other;
+        try {
+            other = std::thread(pass, 1);
+        } catch (...) {   // no thread to be had: both passes on this one
+            pass(1);
+        }
         pass(0);
-        other.join();
+        if (other.joinable()) other.join();
     } else {
         pass((strands & 1) ? 0 : 1);
     }

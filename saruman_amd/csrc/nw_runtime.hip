@@ -421,7 +421,13 @@ void par_ranges(unsigned T, size_t n, Fn fn)
     }
     std::vector<std::thread> th;
     th.reserve(T);
-    for (unsigned u = 0; u < T; ++u) th.emplace_back(fn, u, n * u / T, n * (u + 1) / T);
+    for (unsigned u = 0; u < T; ++u) {
+        try {
+            th.emplace_back(fn, u, n * u / T, n * (u + 1) / T);
+        } catch (...) {   // no thread to be had: the range on this one
+            fn(u, n * u / T, n * (u + 1) / T);
+        }
+    }
     for (auto &x : th) x.join();
 }
 

@@ -68,7 +68,13 @@ void par_ranges(unsigned T, size_t n, Fn fn)
     }
     std::vector<std::thread> th;
     th.reserve(T - 1);
-    for (unsigned u = 0; u + 1 < T; ++u) th.emplace_back(fn, u, n * u / T, n * (u + 1) / T);
+    for (unsigned u = 0; u + 1 < T; ++u) {
+        try {
+            th.emplace_back(fn, u, n * u / T, n * (u + 1) / T);
+        } catch (...) {   // no thread to be had: the range on this one
+            fn(u, n * u / T, n * (u + 1) / T);
+        }
+    }
     fn(T - 1, n * (T - 1) / T, n);
     for (auto &x : th) x.join();
 }

@@ -62,18 +62,22 @@ def test_mix_equals_alone(data):
     assert len({s["slot"] for _, s in mix}) > 1
 
 
-def test_request_matches_oracle_pipeline(data, orc):
+@pytest.mark.parametrize("lo,hi,device", [(20_000, 23_000, False), (20_000, 36_000, True)])
+def test_request_matches_oracle_pipeline(data, orc, lo, hi, device):
     """One request through the server vs filter -> oracle alignment -> oracle
-    records, compared as multisets (record order differs between pipelines)."""
-    print("[gpu] serve vs oracle", flush=True)
+    records, compared as multisets (record order differs between pipelines).  The
+    16,000-read request (2.3 MB of bases, device filter and records) takes the
+    server's multi-threaded host passes: reverse complements, code packing,
+    perfect-hit lines and the record text's copy-out."""
+    print(f"[gpu] serve vs oracle: reads {lo}..{hi}, device {device}", flush=True)
     from saruman_amd.batch import Batch
     genome, pool = data
-    lo, hi = 20_000, 23_000
     sub = wl.Reads(pool.seq, pool.off[lo:hi], pool.length[lo:hi], pool.true_pos[lo:hi],
                    pool.strand[lo:hi], pool.k[lo:hi])
     # request-local read ids and names r<i>
     qs = serve.qs_for_lengths(36, 250)
-    with serve.Server(genome, qs, n_slots=2, chunk_value=5000) as S:
+    with serve.Server(genome, qs, n_slots=2, chunk_value=5000, gpu_filter=device,
+                      gpu_records=device) as S:
         i = S.submit(pool, lo, hi)
         S.wait(i)
         got, st = S.result(i)
