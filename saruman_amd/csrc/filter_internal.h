@@ -152,8 +152,19 @@ srm_filter *batch_pass(const Acc &ro, size_t n_reads, const uint16_t *read_len,
         next_with[r] = (r + 1 < n_reads && ro.n_ent(r + 1) != 0) ? r + 1 : next_with[r + 1];
     // live flag of entry i of read r at live[base[r] + i]
     std::vector<size_t> base(n_reads + 1, 0);
-    for (size_t r = 0; r < n_reads; ++r) base[r + 1] = base[r] + ro.n_ent(r);
+    size_t n_perf_all = 0;
+    for (size_t r = 0; r < n_reads; ++r) {
+        base[r + 1] = base[r] + ro.n_ent(r);
+        n_perf_all += ro.n_perf(r);
+    }
     std::vector<uint8_t> live(base[n_reads], 0);
+    // an entry becomes a candidate at most once: room for all of them up front
+    F->c_read.reserve(base[n_reads]);
+    F->c_pos.reserve(base[n_reads]);
+    F->c_flag.reserve(base[n_reads]);
+    F->c_w.reserve(base[n_reads]);
+    F->p_read.reserve(n_perf_all);
+    F->p_pos.reserve(n_perf_all);
     std::vector<size_t> sticky;
     uint64_t queue = 0;
     size_t pending_lo = 0;
