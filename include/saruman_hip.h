@@ -287,7 +287,8 @@ int srm_job_upload(srm_job *job, int match, int mismatch, const srm_cand *cand, 
                    const uint32_t *read_words, size_t n_read_words, const uint32_t *win_words,
                    size_t n_win_words, const char *read_bytes, size_t n_read_bytes,
                    const char *win_bytes, size_t n_win_bytes);
-/* launch the alignment of every candidate on the context's stream; asynchronous */
+/* launch the alignment of every candidate on the context's stream and wait for it
+ * (its device time is then srm_job_last_ms) */
 int srm_job_run(srm_job *job, int kernel);
 /* wait for the job; copy results (n) and transcript words (ops, may be NULL) */
 int srm_job_download(srm_job *job, srm_result *res, uint32_t *ops, size_t n_ops_words);
