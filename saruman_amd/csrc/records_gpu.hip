@@ -566,10 +566,25 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
             // ---- stage
             if (act) {
                 if (endp > 0) {
+                    // every load first, then the LDS stores, so that the line's loads
+                    // are in flight together (each array holds at most 21 words for a
+                    // line of at most MAXP positions: three words per lane)
                     const uint32_t *ops = a.ops + ops_off;
-                    for (int k = u; k < nw_ops; k += TPL) sops[k] = ops[wlo + k];
-                    for (int k = u; k < rnw; k += TPL) src_[k] = a.read_words[rlo + k];
-                    for (int k = u; k < gnw; k += TPL) sgc[k] = a.win_words[glo + k];
+                    uint32_t xo[3], xr[3], xg[3];
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const int k = u + TPL * t;
+                        xo[t] = k < nw_ops ? ops[wlo + k] : 0u;
+                        xr[t] = k < rnw ? a.read_words[rlo + k] : 0u;
+                        xg[t] = k < gnw ? a.win_words[glo + k] : 0u;
+                    }
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const int k = u + TPL * t;
+                        if (k < nw_ops) sops[k] = xo[t];
+                        if (k < rnw) src_[k] = xr[t];
+                        if (k < gnw) sgc[k] = xg[t];
+                    }
                 }
                 if (NAMES) {
                     const char *nm = a.names + (int64_t)lid * a.name_stride;
