@@ -750,8 +750,18 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                 const int end = al + (int)llen;
                 const int last = end >> 2;   // the word of the line's end
                 const uint32_t *t32 = s_text[wv][q];
-                // whole words [al ? 1 : 0, last)
-                for (int i = (al ? 1 : 0) + u; i < last; i += TPL) o32[i] = t32[i];
+                // whole words [al ? 1 : 0, last), four per lane at a time (the LDS
+                // reads before the stores)
+                int i = (al ? 1 : 0) + u;
+                for (; i + 3 * TPL < last; i += 4 * TPL) {
+                    const uint32_t v0 = t32[i], v1 = t32[i + TPL], v2 = t32[i + 2 * TPL],
+                                   v3 = t32[i + 3 * TPL];
+                    o32[i] = v0;
+                    o32[i + TPL] = v1;
+                    o32[i + 2 * TPL] = v2;
+                    o32[i + 3 * TPL] = v3;
+                }
+                for (; i < last; i += TPL) o32[i] = t32[i];
                 // the first and last words, shared with the neighbouring lines: bytes
                 // [al, 4) of word 0 when al > 0, bytes of word last below end
                 const bool first = u == 0 && al;
