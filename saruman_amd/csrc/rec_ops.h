@@ -59,10 +59,10 @@ struct Measure {
 // visits positions lead, lead + 1, .. while i < w - trail - lead + z (i positions
 // done, z of them gaps), that is while fewer than w - trail - lead DIAG ops have
 // been seen, and stops with eor = i at the row's end.  ld(W) gives word W.  The
-// gap runs at the two ends are read a word at a time (they end in their first
-// word unless it is all TOP ops); the indel scan asks for its words eight at a
-// time, so that a device thread waits for memory once per eight words rather than
-// once per word.
+// top eight words and word 0 are asked for together: they hold the leading and the
+// closing gap runs (each ends in its first word unless that word is all TOP ops)
+// and the indel scan's first words, so that a device thread mostly waits for
+// memory once for all three scans; the indel scan goes on eight words at a time.
 template <class Load>
 SRM_RHD inline Measure measure_ops_by(Load ld, int n, int w)
 {
@@ -70,20 +70,38 @@ SRM_RHD inline Measure measure_ops_by(Load ld, int n, int w)
     Measure m;
     if (n <= 0) return m;   // the scan, if any, meets the NUL at once: eor 0
     const int wtop = (n - 1) >> 4, ftop = (n - 1) & 15;
-    uint32_t x[B];
-    // leading TOP ops from op n - 1 downwards
-    for (int W = wtop, f_hi = ftop; W >= 0; --W, f_hi = 15) {
-        const uint32_t nt = non_tops_of(ld(W)) & field_mask(0, f_hi);
+    uint32_t x[B];   // x[k]: word base - k of the batch at hand
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = wtop - k >= 0 ? ld(wtop - k) : 0u;
+    const uint32_t w0 = ld(0);
+    // leading TOP ops from op n - 1 downwards: the top words first
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+        const int W = wtop - k;
+        if (found || W < 0) continue;
+        const int f_hi = k == 0 ? ftop : 15;
+        const uint32_t nt = non_tops_of(x[k]) & field_mask(0, f_hi);
         if (nt) {
             m.lead += f_hi - (top_bit(nt) >> 1);
-            break;
+            found = true;
+        } else {
+            m.lead += f_hi + 1;
         }
-        m.lead += f_hi + 1;
+    }
+    for (int W = wtop - B; !found && W >= 0; --W) {   // a run of more than 8 words
+        const uint32_t nt = non_tops_of(ld(W)) & 0x55555555u;
+        if (nt) {
+            m.lead += 15 - (top_bit(nt) >> 1);
+            found = true;
+        } else {
+            m.lead += 16;
+        }
     }
     // closing TOP ops from op 0 upwards
     for (int W = 0; W <= wtop; ++W) {
         const int f_hi = W == wtop ? ftop : 15;
-        const uint32_t nt = non_tops_of(ld(W)) & field_mask(0, f_hi);
+        const uint32_t nt = non_tops_of(W == 0 ? w0 : ld(W)) & field_mask(0, f_hi);
         if (nt) {
             m.trail += __builtin_ctz(nt) >> 1;
             break;
@@ -93,15 +111,20 @@ SRM_RHD inline Measure measure_ops_by(Load ld, int n, int w)
     const int need = w - m.trail - m.lead;   // DIAG ops the scan waits for
     const int jtop = n - 1 - m.lead;         // op of position lead
     if (need <= 0 || jtop < 0) return m;     // no pass, or the NUL at once (eor 0)
+    const int jw = jtop >> 4;
     int diag = 0;
-    for (int base = jtop >> 4; base >= 0; base -= B) {
+    // batches of B words from the top word (the first one already loaded); words
+    // above jtop's hold no field of the scan
+    for (int base = wtop; base >= 0; base -= B) {
+        if (base != wtop) {
 #pragma unroll
-        for (int k = 0; k < B; ++k) x[k] = base - k >= 0 ? ld(base - k) : 0u;
+            for (int k = 0; k < B; ++k) x[k] = base - k >= 0 ? ld(base - k) : 0u;
+        }
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             const int W = base - k;
-            if (W < 0) continue;
-            const int f_hi = W == jtop >> 4 ? jtop & 15 : 15;
+            if (W < 0 || W > jw) continue;
+            const int f_hi = W == jw ? jtop & 15 : 15;
             const uint32_t valid = field_mask(0, f_hi);
             const uint32_t d = diags_of(x[k]) & valid;
             const int c = popc32(d);
