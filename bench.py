@@ -54,8 +54,8 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 PMC_PROFILE = [
     {"workload": "mixed", "candidates": 1774238,
      "tuning": {"blocks_per_cu": [0, 0, 0, 0, 0, 0, 0, 0, 0], "merge_below": 2, "class_streams": 0, "tile_cands": 64, "block_waves": 1, "split_len": 0, "walk_layout": 1},
-     "kernel_sha": "92d8c61c78a5", "band_bytes": 1515447425, "band_valu_busy": 0.566,
-     "records_bytes": 1498005480},
+     "kernel_sha": "1152b8e0682e", "band_bytes": 1516833814, "band_valu_busy": 0.565,
+     "records_bytes": 1500855342},
 ]
 
 
