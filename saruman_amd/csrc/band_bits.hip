@@ -642,9 +642,12 @@ void bits_trace_kernel(BandArgs a)
         if (di < 0 || di >= BMAX) return 0u;
         const uint32_t b = (uint32_t)(di & 31);
         const bool hi = NWD == 2 && di >= 32;
+        // bit b of each column word into bit j: a bit-field extract and a shift-or
+        // per column
         uint32_t x = 0;
 #pragma unroll
-        for (int j = 0; j < 32; ++j) x |= ((hi ? dcol[j][NWD - 1] : dcol[j][0]) >> b & 1u) << j;
+        for (int j = 0; j < 32; ++j)
+            x = (__builtin_amdgcn_ubfe(hi ? dcol[j][NWD - 1] : dcol[j][0], b, 1u) << j) | x;
         return x;
     };
     // the walk goes by segments of 32 columns (4 blocks) along its diagonal: the D0

@@ -43,6 +43,14 @@ inline int __builtin_amdgcn_sbfe(int x, uint32_t off, uint32_t w)
     return (int)((int64_t)((uint64_t)(uint32_t)x << (64 - off - w)) >> (64 - w));
 }
 inline uint32_t __builtin_amdgcn_readfirstlane(uint32_t x) { return x; }
+// v_bfe_u32 (unsigned bit-field extract: w bits from bit off, both taken mod 32)
+inline uint32_t __builtin_amdgcn_ubfe(uint32_t x, uint32_t off, uint32_t w)
+{
+    off &= 31;
+    w &= 31;
+    if (w == 0) return 0;
+    return (uint32_t)(((uint64_t)x >> off) & ((1ull << w) - 1ull));
+}
 // a wave vote, taken here over the one lane that runs: a kernel may use it only for
 // choices that are correct for each lane that votes true
 inline bool __all(bool x) { return x; }
