@@ -465,7 +465,7 @@ __device__ __forceinline__ void put_dec(char *o, int32_t v, int nd)
 }
 
 template <bool NAMES>   // names from a table (else prefix + id)
-__global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(5))) rec_write_lines(RecArgs a)
 {
     __shared__ uint32_t s_text[RB / 64][QL][SLOT_TEXT];
     // transcript words staged after one zero word (fields below the staged ones read 0)
@@ -563,41 +563,36 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
             const int rnw = (int)((read_off + (uint32_t)min(end_col + endp, L)) >> 4) - rlo + 1;
             const int gnw = (int)((win_off + (uint32_t)min(end_row + endp, w)) >> 4) - glo + 1;
             uint32_t *sops = s_ops[wv][q] + 1, *src_ = s_rc[wv][q], *sgc = s_gc[wv][q];
-            // ---- stage
-            if (act) {
-                if (endp > 0) {
-                    // every load first, then the LDS stores, so that the line's loads
-                    // are in flight together (each array holds at most 21 words for a
-                    // line of at most MAXP positions: three words per lane)
-                    const uint32_t *ops = a.ops + ops_off;
-                    uint32_t xo[3], xr[3], xg[3];
+            // ---- stage: the loads of the line's transcript, read and window words,
+            // all issued at once; they land in LDS after the header is built, which
+            // needs none of them (each array holds at most 21 words for a line of at
+            // most MAXP positions: three words per lane)
+            const bool staged = act && endp > 0;
+            uint32_t xo[3] = {0u, 0u, 0u}, xr[3] = {0u, 0u, 0u}, xg[3] = {0u, 0u, 0u};
+            if (staged) {
+                const uint32_t *ops = a.ops + ops_off;
 #pragma unroll
-                    for (int t = 0; t < 3; ++t) {
-                        const int k = u + TPL * t;
-                        xo[t] = k < nw_ops ? ops[wlo + k] : 0u;
-                        xr[t] = k < rnw ? a.read_words[rlo + k] : 0u;
-                        xg[t] = k < gnw ? a.win_words[glo + k] : 0u;
-                    }
-#pragma unroll
-                    for (int t = 0; t < 3; ++t) {
-                        const int k = u + TPL * t;
-                        if (k < nw_ops) sops[k] = xo[t];
-                        if (k < rnw) src_[k] = xr[t];
-                        if (k < gnw) sgc[k] = xg[t];
-                    }
+                for (int t = 0; t < 3; ++t) {
+                    const int k = u + TPL * t;
+                    xo[t] = k < nw_ops ? ops[wlo + k] : 0u;
+                    xr[t] = k < rnw ? a.read_words[rlo + k] : 0u;
+                    xg[t] = k < gnw ? a.win_words[glo + k] : 0u;
                 }
-                if (NAMES) {
+            }
+            if (NAMES) {   // a table name goes to LDS first: the header copies it
+                if (act) {
                     const char *nm = a.names + (int64_t)lid * a.name_stride;
                     char *sn = (char *)s_name[NAMES ? wv : 0][NAMES ? q : 0];
                     for (int i = u; i < a.name_stride; i += TPL) sn[i] = nm[i];
                 }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // ---- build
+            // ---- build: the header
             const int al = (int)(loff & 3u);
             char *line = (char *)s_text[wv][q] + al;
+            int hdr = 0;
             if (act) {
                 // the line's four numbers, a lane each: start, stop, the read id (when
                 // names are prefix + id) and the score; their widths to every lane
@@ -613,7 +608,7 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                 } else {
                     name_n = a.prefix_len + ndi;
                 }
-                const int hdr = name_n + 1 + nds + 1 + ndt + 1 + 2 + 1;
+                hdr = name_n + 1 + nds + 1 + ndt + 1 + 2 + 1;
                 // header: name \t start \t stop \t << or >> \t: the name's characters
                 // (those of the prefix when names are prefix + id) over the lanes, the
                 // separators and direction characters a lane each, the digits below
@@ -646,6 +641,22 @@ __global__ void __launch_bounds__(RB) rec_write_lines(RecArgs a)
                     else if (u == 3) at = hdr + 2 * rn + 2;
                     if (at >= 0) put_dec(line + at, num, ndn);
                 }
+            }
+            // ---- stage: the staged words into LDS
+            if (staged) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const int k = u + TPL * t;
+                    if (k < nw_ops) sops[k] = xo[t];
+                    if (k < rnw) src_[k] = xr[t];
+                    if (k < gnw) sgc[k] = xg[t];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // ---- build: the rows
+            if (act) {
                 // the line's positions split over its lanes in runs of a multiple of 4
                 // (at most CHUNK each)
                 const int chunk = (((endp + TPL - 1) / TPL) + 3) & ~3;
