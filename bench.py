@@ -54,8 +54,8 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 PMC_PROFILE = [
     {"workload": "mixed", "candidates": 1774238,
      "tuning": {"blocks_per_cu": [0, 0, 0, 0, 0, 0, 0, 0, 0], "merge_below": 2, "class_streams": 0, "tile_cands": 64, "block_waves": 1, "split_len": 0, "walk_layout": 1},
-     "kernel_sha": "69829f47bc80", "band_bytes": 1516653292, "band_valu_busy": 0.558,
-     "records_bytes": 1500974344},
+     "kernel_sha": "ffb93f872103", "band_bytes": 1515472944, "band_valu_busy": 0.544,
+     "records_bytes": 1501263977},
 ]
 
 
