@@ -3,11 +3,14 @@
 // Same candidates, flags, perfect hits and batch boundaries as the host filter
 // (filter.cpp); only the per-read work moves to the device:
 //   1. count   one thread per read looks up each of its q-grams in the device
-//              copy of the q-gram index and writes the list length + 1;
+//              copy of the q-gram index and writes the list length + 1 and
+//              the index of the list's first key;
 //   2. scan    exclusive sum of the lengths = the offset of every 0-terminated
 //              list in one flat array, reads in order (the reference's
 //              data_array, hash.c:396-461, for all chunks at once);
-//   3. fill    the same threads write the lists;
+//   3. fill    the same threads write the lists: a copy of the key run the
+//              count found, or the hash-table lookup again for a q-gram
+//              holding a byte other than ACGT;
 //   4. pair    one thread per read runs run_seashore (hash.c:502-834) and
 //              add_start_position (hash.c:40-178) over the flat array.  A list
 //              index beyond the list runs on into the following lists and the
@@ -53,9 +56,12 @@ __global__ void __launch_bounds__(FB) fl_count(FArgs a)
     const DQ X = a.qix[P.q];
     const char *rd = a.reads + a.roff[r];
     const uint32_t lb = a.list_base[r];
-    for (int x = 0; x < P.n; ++x)
+    for (int x = 0; x < P.n; ++x) {
+        uint32_t lo;
         a.list_cnt[lb + x] =
-            lookup<false>(X, a.genome, rd, P.L, a.dr, a.gram[P.gram_off + x], nullptr) + 1;
+            lookup<false>(X, a.genome, rd, P.L, a.dr, a.gram[P.gram_off + x], nullptr, &lo) + 1;
+        a.list_lo[lb + x] = lo;
+    }
 }
 
 __global__ void __launch_bounds__(FB) fl_fill(FArgs a)
@@ -66,9 +72,17 @@ __global__ void __launch_bounds__(FB) fl_fill(FArgs a)
     const DQ X = a.qix[P.q];
     const char *rd = a.reads + a.roff[r];
     const uint32_t lb = a.list_base[r];
+    const uint64_t mask = ((uint64_t)1 << X.pos_bits) - 1;
     for (int x = 0; x < P.n; ++x) {
         uint32_t *dst = a.flat + a.list_start[lb + x];
-        const uint32_t c = lookup<true>(X, a.genome, rd, P.L, a.dr, a.gram[P.gram_off + x], dst);
+        const uint32_t lo = a.list_lo[lb + x];
+        uint32_t c;
+        if (lo != NO_RANGE) {   // the positions are the keys fl_count found
+            c = a.list_cnt[lb + x] - 1;
+            for (uint32_t i = 0; i < c; ++i) dst[i] = (uint32_t)(X.keys[lo + i] & mask);
+        } else {
+            c = lookup<true>(X, a.genome, rd, P.L, a.dr, a.gram[P.gram_off + x], dst);
+        }
         dst[c] = 0;
     }
 }
@@ -199,7 +213,7 @@ struct srm_dfilter {
     size_t glen = 0;
     GBuf genome, dq;
     std::vector<GBuf> keys, pre, odd_hash, odd_pos;
-    GBuf par, gram, reads, roff, rlen, pid, list_base, list_cnt, list_start, flat, cub_tmp;
+    GBuf par, gram, reads, roff, rlen, pid, list_base, list_cnt, list_lo, list_start, flat, cub_tmp;
     GBuf slot_pos, slot_flag, slot_perf, out_pos, out_flag, out_perf, meta, counters, todo;
     hipEvent_t ev[2] = {nullptr, nullptr};
     float last_ms = 0.f;
@@ -208,7 +222,7 @@ struct srm_dfilter {
     ~srm_dfilter()
     {
         for (GBuf *b : {&genome, &dq, &par, &gram, &reads, &roff, &rlen, &pid, &list_base,
-                        &list_cnt, &list_start, &flat, &cub_tmp, &slot_pos, &slot_flag,
+                        &list_cnt, &list_lo, &list_start, &flat, &cub_tmp, &slot_pos, &slot_flag,
                         &slot_perf, &out_pos, &out_flag, &out_perf, &meta, &counters, &todo})
             b->release();
         for (auto *v : {&keys, &pre, &odd_hash, &odd_pos})
@@ -315,6 +329,7 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
     FL_RC(G->upload(G->pid, pid.data(), n_reads * 2));
     FL_RC(G->upload(G->list_base, list_base.data(), (n_reads + 1) * 4));
     FL_RC(G->list_cnt.ensure((nl + 1) * 4));
+    FL_RC(G->list_lo.ensure((nl + 1) * 4));
     FL_RC(G->list_start.ensure((nl + 1) * 4));
     FArgs a{};
     a.qix = G->dq.as<DQ>();
@@ -330,6 +345,7 @@ int run_device(srm_dfilter *G, const uint8_t *blocked, const char *reads,
     a.cv = (uint32_t)chunk_value;
     a.list_base = G->list_base.as<uint32_t>();
     a.list_cnt = G->list_cnt.as<uint32_t>();
+    a.list_lo = G->list_lo.as<uint32_t>();
     a.list_start = G->list_start.as<uint32_t>();
     // per strand: the device's entries and perfect hits for the batch pass, which
     // runs on the host once both strands' device work is done (the two strands'

@@ -52,6 +52,7 @@ struct FArgs {
     uint32_t cv;             // chunk_value
     const uint32_t *list_base;   // [n_reads + 1] first list of every read
     uint32_t *list_cnt;          // [n_lists + 1] list length + 1 (terminator)
+    uint32_t *list_lo;           // [n_lists] first key of the list (NO_RANGE: search again)
     const uint32_t *list_start;  // [n_lists + 1] exclusive sum of list_cnt
     uint32_t *flat;
     // pairing
@@ -104,25 +105,32 @@ SRM_HD inline uint64_t fnv_step(uint64_t h, unsigned char c)
 }
 constexpr uint64_t FNV_BASIS = 14695981039346656037ull;
 
+// list_lo of a q-gram whose positions are not one run of keys (it holds a byte
+// other than ACGT: its positions come from the hash table and a byte compare)
+constexpr uint32_t NO_RANGE = 0xFFFFFFFFu;
+
 // positions of the q-gram at read offset g, ascending; writes them to out when
-// FILL, returns their number
+// FILL, returns their number; lo_out (when given) receives the index of the first
+// key (the positions are keys lo .. lo + count - 1), or NO_RANGE for a q-gram
+// holding another byte
 template <bool FILL>
 SRM_HD uint32_t lookup(const DQ &X, const char *genome, const char *rd, int L, int dr, int g,
-                       uint32_t *out)
+                       uint32_t *out, uint32_t *lo_out = nullptr)
 {
     const int q = X.q;
     uint64_t code = 0;
     bool odd = false;
+    // all q bytes, with no exit from the loop (on gfx950 a loop that left at the
+    // first other byte gave fl_count wrong list lengths for some lanes once the key
+    // range was stored as well; the same source on the host gave the right ones)
     for (int i = 0; i < q; ++i) {
         const int c = dcode(obyte(rd, L, dr, g + i));
-        if (c < 0) {
-            odd = true;
-            break;
-        }
-        code = (code << 2) | (uint64_t)c;
+        odd |= c < 0;
+        code = (code << 2) | (uint64_t)(c & 3);
     }
     uint32_t cnt = 0;
     if (odd) {
+        if (lo_out) *lo_out = NO_RANGE;
         uint64_t h = FNV_BASIS;
         for (int i = 0; i < q; ++i) h = fnv_step(h, (unsigned char)obyte(rd, L, dr, g + i));
         uint32_t lo = 0, hi = X.n_odd;
@@ -150,6 +158,7 @@ SRM_HD uint32_t lookup(const DQ &X, const char *genome, const char *rd, int L, i
         if (X.keys[mid] < key) lo = mid + 1;
         else hi = mid;
     }
+    if (lo_out) *lo_out = lo;
     const uint64_t mask = ((uint64_t)1 << X.pos_bits) - 1;
     const uint32_t end = X.pre[p + 1];
     for (uint32_t i = lo; i < end && (X.keys[i] >> X.pos_bits) == code; ++i) {

@@ -99,7 +99,7 @@ struct Emu {
         std::vector<uint16_t> gram;
         device_params(plan.params, dpar, gram);
         const uint64_t nl = plan.list_base[n];
-        std::vector<uint32_t> list_cnt(nl + 1), list_start(nl + 1);
+        std::vector<uint32_t> list_cnt(nl + 1), list_start(nl + 1), list_lo(nl + 1);
         FArgs a{};
         a.qix = dq.data();
         a.par = dpar.data();
@@ -115,15 +115,24 @@ struct Emu {
         a.cv = (uint32_t)cv;
         a.list_base = plan.list_base.data();
         a.list_cnt = list_cnt.data();
+        a.list_lo = list_lo.data();
         a.list_start = list_start.data();
-        // count (fl_count)
+        // count (fl_count): the list lengths and key ranges, checked against lookup
         for (uint32_t r = 0; r < n; ++r) {
             const DParam P = a.par[a.pid[r]];
             const DQ X = a.qix[P.q];
-            for (int x = 0; x < P.n; ++x)
-                list_cnt[a.list_base[r] + x] =
-                    lookup<false>(X, a.genome, reads + roff[r], P.L, dr, a.gram[P.gram_off + x],
-                                  nullptr) + 1;
+            for (int x = 0; x < P.n; ++x) {
+                uint32_t lo;
+                const uint32_t c = lookup<false>(X, a.genome, reads + roff[r], P.L, dr,
+                                                 a.gram[P.gram_off + x], nullptr, &lo);
+                if (c != lookup<false>(X, a.genome, reads + roff[r], P.L, dr,
+                                       a.gram[P.gram_off + x], nullptr)) {
+                    fprintf(stderr, "lookup count with the key range differs (read %u list %d)\n", r, x);
+                    exit(1);
+                }
+                list_cnt[a.list_base[r] + x] = c + 1;
+                list_lo[a.list_base[r] + x] = lo;
+            }
         }
         list_cnt[nl] = 0;
         std::exclusive_scan(list_cnt.begin(), list_cnt.end(), list_start.begin(), 0u);
@@ -133,10 +142,19 @@ struct Emu {
         for (uint32_t r = 0; r < n; ++r) {
             const DParam P = a.par[a.pid[r]];
             const DQ X = a.qix[P.q];
+            const uint64_t mask = ((uint64_t)1 << X.pos_bits) - 1;
             for (int x = 0; x < P.n; ++x) {
                 uint32_t *dst = a.flat + list_start[a.list_base[r] + x];
                 const uint32_t c = lookup<true>(X, a.genome, reads + roff[r], P.L, dr,
                                                 a.gram[P.gram_off + x], dst);
+                // fl_fill copies the keys of the range fl_count found: the same positions
+                const uint32_t lo = list_lo[a.list_base[r] + x];
+                if (lo != NO_RANGE)
+                    for (uint32_t i = 0; i < c; ++i)
+                        if ((uint32_t)(X.keys[lo + i] & mask) != dst[i]) {
+                            fprintf(stderr, "key range differs (read %u list %d)\n", r, x);
+                            exit(1);
+                        }
                 dst[c] = 0;
             }
         }
