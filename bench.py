@@ -54,7 +54,7 @@ OPS_PER_BAND_CELL = 6       # compare + select (substitution), 3 adds, min3
 PMC_PROFILE = [
     {"workload": "mixed", "candidates": 1774238,
      "tuning": {"blocks_per_cu": [0, 0, 0, 0, 0, 0, 0, 0, 0], "merge_below": 2, "class_streams": 0, "tile_cands": 64, "block_waves": 1, "split_len": 0, "walk_layout": 1},
-     "kernel_sha": "ffb93f872103", "band_bytes": 1515472944, "band_valu_busy": 0.544,
+     "kernel_sha": "05feeb9780ba", "band_bytes": 1515472944, "band_valu_busy": 0.544,
      "records_bytes": 1501263977},
 ]
 

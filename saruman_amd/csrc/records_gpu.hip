@@ -3,12 +3,16 @@
 // After a job has run, every hit (score <= k) becomes the TSV line of the
 // reference's post-processing (src/cuda-functions.cu:828-994), byte for byte the
 // line srm_format_record writes on the host:
-//   rec_measure  one thread per candidate derives, from the transcript alone,
-//                the leading and closing gaps of the read row, the indel scan,
-//                the printed length and the start/stop, and the line length;
-//   (scan)       exclusive sum of the line lengths = each line's offset;
-//   rec_write    the same thread replays the transcript from the walk's end cell
-//                and writes name, coordinates, both rows and the score.
+//   rec_measure      one thread per candidate derives, from the transcript
+//                    alone, the leading and closing gaps of the read row, the
+//                    indel scan, the printed length and the start/stop, and the
+//                    line length;
+//   (scan)           exclusive sum of the line lengths = each line's offset;
+//   rec_write_lines  eight lines per wave, eight lanes per line, for the lines of
+//                    up to 320 printed positions: name, coordinates, both rows and
+//                    the score, built in LDS and copied out by whole words;
+//   rec_write        one wave per line for the rest (long lines, raw-byte
+//                    candidates).
 // The text lands in candidate order in one HBM buffer that the host copies out
 // in one transfer.
 #include <hip/hip_runtime.h>
