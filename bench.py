@@ -472,10 +472,10 @@ def main():
     # copied back, transfers overlapped with the other lane's device work
     h2h = None
     if not args.no_h2h:
-        reps = max(1, 3 // B) if B < 3 else 1
+        reps = max(1, 8 // B) if B < 8 else 1
         pinned = [PinnedBatch.of(db, lo) for lo, hi, db in batches]
         order = [pinned[i % B] for i in range(B * reps)]
-        with BatchPipeline(wl.genome_words(genome), lanes=2) as pipe:
+        with BatchPipeline(wl.genome_words(genome), lanes=2, tuning=tuning) as pipe:
             pipe.run(order[:min(2, len(order))])            # untimed: lane buffers, graphs
             h_wall, h_res, _ = pipe.run(order)
         for p in pinned:

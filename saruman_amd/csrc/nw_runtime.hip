@@ -144,11 +144,38 @@ struct PlainChunk {
     uint32_t begin = 0, count = 0;   // range in the plain index list
     size_t mat_elems = 0;
 };
+// one host thread's share of an upload's validation and classification
+struct UploadPart {
+    std::vector<uint32_t> band, rest, bytes;
+    size_t ops_end = 0, rows = 0;
+    uint32_t maxL = 0;   // over the banded candidates
+    int bad = -1, why = 0;
+    void reset()
+    {
+        band.clear();
+        rest.clear();
+        bytes.clear();
+        ops_end = rows = 0;
+        maxL = 0;
+        bad = -1;
+        why = 0;
+    }
+};
+// the host arrays of an upload's plan, kept by the job from one upload to the
+// next: a batch of about the same size then reuses their pages (a fresh
+// multi-megabyte allocation is faulted in page by page, which cost more than the
+// planning itself)
+struct HostPlan {
+    std::vector<UploadPart> part;
+    std::vector<uint32_t> band_idx, rest_idx, bytes_idx, all_idx, tiles;
+    std::vector<uint64_t> kv, kv_tmp, tile_off, off_rest, off_all, off_bytes;
+};
 }  // namespace
 
 struct srm_job {
     srm_ctx *ctx = nullptr;
     int n = 0;
+    HostPlan plan;                   // host arrays of the last upload's plan
     int ma = 0, mm = 1;
     bool unit = true;
     bool resident_genome = false;
@@ -435,14 +462,15 @@ void par_ranges(unsigned T, size_t n, Fn fn)
 // digit counting passes of at most 11 bits, each split over T threads (a thread
 // counts its range's digits, then scatters that range to the offsets its counts
 // give, so equal digits keep their order)
-void radix_sort_kv(std::vector<uint64_t> &kv, int lo, int bits, unsigned T)
+void radix_sort_kv(std::vector<uint64_t> &kv, std::vector<uint64_t> &tmp, int lo, int bits,
+                   unsigned T)
 {
     const size_t n = kv.size();
     const int passes = (bits + 10) / 11;
     if (n < 2 || passes == 0) return;
     const int db = (bits + passes - 1) / passes;
     const size_t B = (size_t)1 << db;
-    std::vector<uint64_t> tmp(n);
+    tmp.resize(n);
     std::vector<size_t> cnt((size_t)T * B);
     for (int pass = 0; pass < passes; ++pass) {
         const int sh = lo + pass * db;
@@ -827,15 +855,12 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         return K_REST;
     };
     const unsigned TV = plan_threads((size_t)n);
-    struct Part {
-        std::vector<uint32_t> band, rest, bytes;
-        size_t ops_end = 0, rows = 0;
-        uint32_t maxL = 0;   // over the banded candidates
-        int bad = -1, why = 0;
-    };
-    std::vector<Part> part(TV);
+    HostPlan &hp = j->plan;
+    std::vector<UploadPart> &part = hp.part;
+    part.resize(TV);
+    for (UploadPart &P : part) P.reset();
     par_ranges(TV, (size_t)n, [&](unsigned u, size_t a, size_t b) {
-        Part &P = part[u];
+        UploadPart &P = part[u];
         for (size_t t = a; t < b; ++t) {
             const srm_cand &c = cand[t];
             const int r = classify(c);
@@ -850,10 +875,14 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             (r == K_BAND ? P.band : r == K_REST ? P.rest : P.bytes).push_back((uint32_t)t);
         }
     });
-    std::vector<uint32_t> band_idx, rest_idx, bytes_idx;
+    std::vector<uint32_t> &band_idx = hp.band_idx, &rest_idx = hp.rest_idx,
+                          &bytes_idx = hp.bytes_idx;
+    band_idx.clear();
+    rest_idx.clear();
+    bytes_idx.clear();
     size_t ops_end = 0;
     j->rec_rows_bound = 0;
-    for (const Part &P : part) {
+    for (const UploadPart &P : part) {
         if (P.bad >= 0) {
             const srm_cand &c = cand[P.bad];
             switch (P.why) {
@@ -875,7 +904,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         rest_idx.swap(part[0].rest);
         bytes_idx.swap(part[0].bytes);
     } else {
-        for (const Part &P : part) {
+        for (const UploadPart &P : part) {
             band_idx.insert(band_idx.end(), P.band.begin(), P.band.end());
             rest_idx.insert(rest_idx.end(), P.rest.begin(), P.rest.end());
             bytes_idx.insert(bytes_idx.end(), P.bytes.begin(), P.bytes.end());
@@ -904,10 +933,11 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     // then maxL - L in lb bits (L descending), the fields of the order above from
     // the top down
     uint32_t maxL = 0;
-    for (const Part &P : part) maxL = std::max(maxL, P.maxL);
+    for (const UploadPart &P : part) maxL = std::max(maxL, P.maxL);
     int lb = 1;
     while ((1u << lb) <= maxL) ++lb;
-    std::vector<uint64_t> kv(nb);
+    std::vector<uint64_t> &kv = hp.kv;
+    kv.resize(nb);
     par_ranges(TB, nb, [&](unsigned, size_t a, size_t b) {
         for (size_t i = a; i < b; ++i) {
             const srm_cand &c = cand[band_idx[i]];
@@ -918,7 +948,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         }
     });
     static_assert(SRM_BAND_KMAX < 16, "k of a banded candidate fits 4 bits");
-    radix_sort_kv(kv, 36, 8 + lb, TB);
+    radix_sort_kv(kv, hp.kv_tmp, 36, 8 + lb, TB);
     par_ranges(TB, nb, [&](unsigned, size_t a, size_t b) {
         for (size_t i = a; i < b; ++i) band_idx[i] = (uint32_t)kv[i];
     });
@@ -927,10 +957,12 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     auto lmod_at = [&](size_t i) { return (int)((kv[i] >> (36 + lb)) & 7u); };
     auto k_at = [&](size_t i) { return (uint32_t)(kv[i] >> 32) & 15u; };
     auto L_at = [&](size_t i) { return maxL - ((uint32_t)(kv[i] >> 36) & ((1u << lb) - 1u)); };
-    std::vector<uint32_t> tiles;   // uint4 per tile
+    std::vector<uint32_t> &tiles = hp.tiles;   // uint4 per tile
+    tiles.clear();
     j->classes.clear();
     size_t scratch_words = 0;
-    std::vector<uint64_t> tile_off;   // word offset of each tile's band
+    std::vector<uint64_t> &tile_off = hp.tile_off;   // word offset of each tile's band
+    tile_off.clear();
     // band classes run side by side on 4 streams unless the tuning says otherwise
     // (1 = one after another on the context's stream)
     const int want_streams = ctx->tuning.class_streams == 0 ? 4 : ctx->tuning.class_streams;
@@ -1011,7 +1043,8 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
 
     // ---- plain plans
     // every candidate that is not on the raw-byte path, in index order
-    std::vector<uint32_t> all_idx;
+    std::vector<uint32_t> &all_idx = hp.all_idx;
+    all_idx.clear();
     all_idx.reserve(band_idx.size() + rest_idx.size());
     {
         size_t b = 0;
@@ -1053,7 +1086,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     // srm_job_bytes_bound reserves.  A chunk holds at least one candidate.  The
     // matrix buffer itself is allocated by srm_job_run, for the path it runs: a
     // banded run whose candidates all fit the band needs none.
-    std::vector<uint64_t> off_rest, off_all, off_bytes;
+    std::vector<uint64_t> &off_rest = hp.off_rest, &off_all = hp.off_all, &off_bytes = hp.off_bytes;
     size_t budget = SRM_PLAIN_CHUNK;
     if (!ctx->pool_cap) {
         size_t free_b = 0, total_b = 0;

@@ -66,9 +66,13 @@ class LaneResult:
 
 class BatchPipeline:
     def __init__(self, genome_words: np.ndarray, lanes: int = 2, device: int = 0,
-                 graph: bool = True, prefix: str = "r"):
+                 graph: bool = True, prefix: str = "r", tuning: dict | None = None):
         self.lib = _lib.load()
         self.ctxs = [Context(device) for _ in range(max(1, lanes))]
+        if tuning is not None:          # the launch tuning of every lane (tune.apply)
+            from . import tune
+            for c in self.ctxs:
+                tune.apply(c, tuning)
         self.ctxs[0].set_genome(genome_words)
         for c in self.ctxs[1:]:
             _lib.check(self.lib.srm_ctx_borrow_genome(c.h, self.ctxs[0].h),
