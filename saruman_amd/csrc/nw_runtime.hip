@@ -169,6 +169,7 @@ struct HostPlan {
     std::vector<UploadPart> part;
     std::vector<uint32_t> band_idx, rest_idx, bytes_idx, all_idx, tiles;
     std::vector<uint64_t> kv, kv_tmp, tile_off, off_rest, off_all, off_bytes;
+    std::vector<uint32_t> mat;   // per candidate: its plain-kernel matrix elements
 };
 }  // namespace
 
@@ -176,6 +177,7 @@ struct srm_job {
     srm_ctx *ctx = nullptr;
     int n = 0;
     HostPlan plan;                   // host arrays of the last upload's plan
+    bool all_planned = false;        // the plain-kernel plan of every candidate made
     int ma = 0, mm = 1;
     bool unit = true;
     bool resident_genome = false;
@@ -248,7 +250,15 @@ int upload(srm_ctx *ctx, DevBuf &b, const T *src, size_t count)
 }
 
 // builds chunked matrix offsets for a plain index list
-void plan_plain(const std::vector<uint32_t> &idx, const srm_cand *cand, size_t budget_elems,
+// the plain kernel's int16 matrix of a candidate, in elements, 128-byte aligned
+inline uint32_t plain_mat_elems(const srm_cand &c)
+{
+    return (uint32_t)((((size_t)c.w + 1) * ((size_t)c.L + 1) + 63) & ~(size_t)63);
+}
+
+// matrix offsets and chunks of the candidates idx (mat: every candidate's matrix
+// elements)
+void plan_plain(const std::vector<uint32_t> &idx, const uint32_t *mat, size_t budget_elems,
                 std::vector<uint64_t> &off, std::vector<PlainChunk> &chunks)
 {
     off.resize(idx.size());
@@ -256,9 +266,7 @@ void plan_plain(const std::vector<uint32_t> &idx, const srm_cand *cand, size_t b
     PlainChunk cur;
     size_t acc = 0;
     for (size_t j = 0; j < idx.size(); ++j) {
-        const srm_cand &c = cand[idx[j]];
-        size_t m = (size_t)(c.w + 1) * (size_t)(c.L + 1);
-        m = (m + 63) & ~(size_t)63;   // 128-byte aligned matrices
+        const size_t m = mat[idx[j]];
         if (cur.count && acc + m > budget_elems) {
             cur.mat_elems = acc;
             chunks.push_back(cur);
@@ -274,6 +282,42 @@ void plan_plain(const std::vector<uint32_t> &idx, const srm_cand *cand, size_t b
         cur.mat_elems = acc;
         chunks.push_back(cur);
     }
+}
+
+size_t max_chunk(std::initializer_list<const std::vector<PlainChunk> *> vs)
+{
+    size_t m = 0;
+    for (auto *v : vs)
+        for (auto &ch : *v) m = std::max(m, ch.mat_elems);
+    return m;
+}
+
+// the plain kernel's plan over every candidate not on the raw-byte path, in index
+// order, and its device lists: made for the first plain run after an upload (a
+// banded run never needs it)
+int plan_all(srm_job *j)
+{
+    if (j->all_planned) return SRM_OK;
+    HostPlan &hp = j->plan;
+    std::vector<uint32_t> &all_idx = hp.all_idx;
+    all_idx.clear();
+    all_idx.reserve((size_t)j->n - hp.bytes_idx.size());
+    size_t b = 0;
+    for (uint32_t t = 0; t < (uint32_t)j->n; ++t) {
+        if (b < hp.bytes_idx.size() && hp.bytes_idx[b] == t) {
+            ++b;
+            continue;
+        }
+        all_idx.push_back(t);
+    }
+    plan_plain(all_idx, hp.mat.data(), j->mat_budget_elems, hp.off_all, j->chunks_all);
+    int rc;
+    if ((rc = upload(j->ctx, j->d_idx_all, all_idx.data(), all_idx.size()))) return rc;
+    if ((rc = upload(j->ctx, j->d_off_all, hp.off_all.data(), hp.off_all.size()))) return rc;
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    j->mat_elems_plain = max_chunk({&j->chunks_all, &j->chunks_bytes});
+    j->all_planned = true;
+    return SRM_OK;
 }
 
 // the record kernels' arguments over the job's buffers (set_up: the record stage
@@ -859,6 +903,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     std::vector<UploadPart> &part = hp.part;
     part.resize(TV);
     for (UploadPart &P : part) P.reset();
+    hp.mat.resize((size_t)n);
     par_ranges(TV, (size_t)n, [&](unsigned u, size_t a, size_t b) {
         UploadPart &P = part[u];
         for (size_t t = a; t < b; ++t) {
@@ -869,6 +914,7 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
                 P.why = r;
                 return;
             }
+            hp.mat[t] = plain_mat_elems(c);
             P.ops_end = std::max(P.ops_end, (size_t)c.ops_off + srm_ops_words(c.w, c.k));
             P.rows += 2 * ((size_t)c.w + c.k);
             if (r == K_BAND) P.maxL = std::max<uint32_t>(P.maxL, c.L);
@@ -1041,21 +1087,6 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         }
     }
 
-    // ---- plain plans
-    // every candidate that is not on the raw-byte path, in index order
-    std::vector<uint32_t> &all_idx = hp.all_idx;
-    all_idx.clear();
-    all_idx.reserve(band_idx.size() + rest_idx.size());
-    {
-        size_t b = 0;
-        for (uint32_t t = 0; t < (uint32_t)n; ++t) {
-            if (b < bytes_idx.size() && bytes_idx[b] == t) {
-                ++b;
-                continue;
-            }
-            all_idx.push_back(t);
-        }
-    }
     // ---- device buffers
     int rc;
     if ((rc = upload(ctx, j->d_cand, cand, (size_t)n))) return rc;
@@ -1074,7 +1105,6 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     if ((rc = upload(ctx, j->d_tiles, tiles.data(), tiles.size()))) return rc;
     if ((rc = upload(ctx, j->d_tile_scratch, tile_off.data(), tile_off.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_rest, rest_idx.data(), rest_idx.size()))) return rc;
-    if ((rc = upload(ctx, j->d_idx_all, all_idx.data(), all_idx.size()))) return rc;
     if ((rc = upload(ctx, j->d_idx_bytes, bytes_idx.data(), bytes_idx.size()))) return rc;
     if ((rc = ctx->ensure(j->d_queue, 64 * sizeof(uint32_t)))) return rc;
     if ((rc = ctx->ensure(j->d_scratch, std::max<size_t>(scratch_words * 4, 256)))) return rc;
@@ -1086,7 +1116,9 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
     // srm_job_bytes_bound reserves.  A chunk holds at least one candidate.  The
     // matrix buffer itself is allocated by srm_job_run, for the path it runs: a
     // banded run whose candidates all fit the band needs none.
-    std::vector<uint64_t> &off_rest = hp.off_rest, &off_all = hp.off_all, &off_bytes = hp.off_bytes;
+    // (the plan of the plain kernel over every candidate waits for a run of it:
+    // plan_all)
+    std::vector<uint64_t> &off_rest = hp.off_rest, &off_bytes = hp.off_bytes;
     size_t budget = SRM_PLAIN_CHUNK;
     if (!ctx->pool_cap) {
         size_t free_b = 0, total_b = 0;
@@ -1094,19 +1126,13 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
         budget = std::max<size_t>(std::min<size_t>(free_b / 4, (size_t)32 << 30), SRM_PLAIN_CHUNK);
     }
     j->mat_budget_elems = budget / 2;
-    plan_plain(rest_idx, cand, j->mat_budget_elems, off_rest, j->chunks_rest);
-    plan_plain(all_idx, cand, j->mat_budget_elems, off_all, j->chunks_all);
-    plan_plain(bytes_idx, cand, j->mat_budget_elems, off_bytes, j->chunks_bytes);
-    auto max_chunk = [](std::initializer_list<const std::vector<PlainChunk> *> vs) {
-        size_t m = 0;
-        for (auto *v : vs)
-            for (auto &ch : *v) m = std::max(m, ch.mat_elems);
-        return m;
-    };
+    plan_plain(rest_idx, hp.mat.data(), j->mat_budget_elems, off_rest, j->chunks_rest);
+    plan_plain(bytes_idx, hp.mat.data(), j->mat_budget_elems, off_bytes, j->chunks_bytes);
     j->mat_elems_band = max_chunk({&j->chunks_rest, &j->chunks_bytes});
-    j->mat_elems_plain = max_chunk({&j->chunks_all, &j->chunks_bytes});
+    j->chunks_all.clear();
+    j->mat_elems_plain = 0;
+    j->all_planned = false;
     if ((rc = upload(ctx, j->d_off_rest, off_rest.data(), off_rest.size()))) return rc;
-    if ((rc = upload(ctx, j->d_off_all, off_all.data(), off_all.size()))) return rc;
     if ((rc = upload(ctx, j->d_off_bytes, off_bytes.data(), off_bytes.size()))) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     j->uploaded = true;
@@ -1118,6 +1144,10 @@ int srm_job_run(srm_job *j, int kernel)
     if (!j->uploaded) return fail(SRM_ESTATE, "srm_job_run before a successful upload");
     srm_ctx *ctx = j->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
+    if (kernel == SRM_KERNEL_PLAIN) {   // its plan over every candidate, on its first run
+        const int rc = plan_all(j);
+        if (rc) return rc;
+    }
     // the plain kernel's matrices for the candidates this run sends there (before
     // any capture: a graph holds the buffer's address)
     const size_t mat = kernel == SRM_KERNEL_PLAIN ? j->mat_elems_plain : j->mat_elems_band;
