@@ -25,8 +25,8 @@ from .workload import DeviceBatch, Pinned
 
 @dataclass
 class PinnedBatch:
-    """A DeviceBatch's upload arrays copied into page-locked memory, with the
-    record inputs (window positions and read ids)."""
+    """A DeviceBatch's upload arrays and record inputs (window positions and read
+    ids) copied into page-locked memory."""
     n: int
     cand: np.ndarray
     read_words: np.ndarray
@@ -44,8 +44,13 @@ class PinnedBatch:
         wb = Pinned(db.read_words.nbytes)
         rw = wb.array[:db.read_words.nbytes].view(np.uint32)
         rw[:] = db.read_words
-        return cls(db.n, cand, rw, db.ops_total, np.ascontiguousarray(db.pos, np.int64),
-                   np.ascontiguousarray(db.read_id + read_id_base, np.int32), n_reads, (cb, wb))
+        pbuf = Pinned(max(db.n, 1) * 8)
+        pos = pbuf.array[:db.n * 8].view(np.int64)
+        pos[:] = db.pos
+        ibuf = Pinned(max(db.n, 1) * 4)
+        rid = ibuf.array[:db.n * 4].view(np.int32)
+        rid[:] = np.asarray(db.read_id, np.int64) + read_id_base
+        return cls(db.n, cand, rw, db.ops_total, pos, rid, n_reads, (cb, wb, pbuf, ibuf))
 
     def free(self):
         for b in self._bufs:
