@@ -310,7 +310,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
-              "(candidates are independent; multi-GPU sharding is described in DESIGN.md)",
+              "(the candidates are independent: several GPUs would each take a share of the batches)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -149,7 +149,8 @@ extern float alignment_time[2];                    /* cuda-functions.cu:61 (ms, 
 #define SRM_CAND_STOP_ROW0 1u   /* reproduce the first-candidate traceback stop (Q1) */
 #define SRM_CAND_BYTES 2u       /* offsets are byte offsets into the byte arrays */
 
-/* limits (documented in DESIGN.md) */
+/* limits: read and window lengths, the banded kernels' band height and k, the plain
+ * matrices' chunk under a pool cap */
 #define SRM_MAX_L 1000
 #define SRM_MAX_W 900
 #define SRM_BAND_MAX 64      /* widest band the banded kernel holds in registers */

@@ -12,8 +12,7 @@
  *   - host post-processing src/cuda-functions.cu:828-1006 (start/stop correction,
  *                          indel count, printf record)
  *
- * Canonical decisions for the layout artefacts of the reference (documented in
- * DESIGN.md's quirk table):
+ * Canonical decisions for the layout artefacts of the reference:
  *   Q1  only the first candidate of a launch stops its traceback on reaching row 0
  *       (cuda-functions.cu:129).  Canonical = the behaviour of every other candidate
  *       (continue along row 0); `stop_at_row0` reproduces the first-candidate case.
