@@ -270,8 +270,8 @@ int srm_band_occupancy(int nw, int unit);
  */
 int srm_job_create(srm_ctx *ctx, srm_job **out);
 void srm_job_destroy(srm_job *job);
-/* frees the job's device buffers (after waiting for its work); the next upload
- * allocates them again */
+/* frees the job's device buffers (after waiting for its work) and the host arrays
+ * an upload keeps for the next one; the next upload allocates them again */
 int srm_job_release(srm_job *job);
 /* Upper bound on the device bytes a job of ctx (NULL: default tuning) needs for
  * these candidates: its uploads, band scratch, results and transcripts, a chunk of

@@ -737,6 +737,8 @@ int srm_job_release(srm_job *j)
     for (hipStream_t q : j->side) HIP_TRY(hipStreamSynchronize(q));
     destroy_graph(j);
     release_job_buffers(j);
+    j->plan = HostPlan();   // and the host arrays an upload keeps for the next one
+    j->all_planned = false;
     j->uploaded = false;
     return SRM_OK;
 }
@@ -1027,23 +1029,32 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             bc.nw = ci / 2;
             bc.tile_begin = (uint32_t)(tiles.size() / 4);
             bc.maxL = 0;
-            size_t e = i;
-            while (e < band_idx.size() && cls_at(e) == ci) ++e;
+            // the sort words are in key order: the class's end and each run's end by
+            // binary search
+            const size_t e = (size_t)(std::partition_point(
+                                          kv.begin() + (std::ptrdiff_t)i, kv.end(),
+                                          [&](const uint64_t &x) {
+                                              return (int)(x >> (36 + lb + 3)) == ci;
+                                          }) -
+                                      kv.begin());
             for (size_t g = i; g < e;) {
-                // one run of equal L mod 8
-                size_t ge = g;
-                while (ge < e && lmod_at(ge) == lmod_at(g)) ++ge;
+                // one run of equal L mod 8, L descending: a tile's first read is its
+                // longest (its k maximum is filled in below, over all tiles at once)
+                const int lm = lmod_at(g);
+                const size_t ge = (size_t)(std::partition_point(
+                                               kv.begin() + (std::ptrdiff_t)g,
+                                               kv.begin() + (std::ptrdiff_t)e,
+                                               [&](const uint64_t &x) {
+                                                   return (int)((x >> (36 + lb)) & 7u) == lm;
+                                               }) -
+                                           kv.begin());
                 for (size_t s = g; s < ge; s += tile_cands) {
                     const uint32_t cnt = (uint32_t)std::min<size_t>(tile_cands, ge - s);
-                    uint32_t kmax = 0, lmax = 0;
-                    for (size_t u = s; u < s + cnt; ++u) {
-                        kmax = std::max(kmax, k_at(u));
-                        lmax = std::max(lmax, L_at(u));
-                    }
+                    const uint32_t lmax = L_at(s);
                     tiles.push_back((uint32_t)s);
                     tiles.push_back(cnt);
                     tiles.push_back(lmax);
-                    tiles.push_back(kmax);
+                    tiles.push_back(0u);
                     bc.maxL = std::max(bc.maxL, (int)lmax);
                 }
                 g = ge;
@@ -1085,6 +1096,18 @@ int srm_job_upload(srm_job *j, int match, int mismatch, const srm_cand *cand, in
             j->classes.push_back(bc);
             i = e;
         }
+    }
+    // every tile's largest k, over the tiles on several host threads
+    {
+        const size_t nt = tiles.size() / 4;
+        par_ranges(plan_threads(nt * tile_cands), nt, [&](unsigned, size_t a, size_t b) {
+            for (size_t t = a; t < b; ++t) {
+                const size_t s0 = tiles[4 * t], c0 = tiles[4 * t + 1];
+                uint32_t kmax = 0;
+                for (size_t u = s0; u < s0 + c0; ++u) kmax = std::max(kmax, k_at(u));
+                tiles[4 * t + 3] = kmax;
+            }
+        });
     }
 
     // ---- device buffers
